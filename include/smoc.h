@@ -1,0 +1,209 @@
+/*
+ * smoc.h -- C ABI of libsmoc.so, the MI355X (gfx950) segment-attenuation
+ * library for SimpleMOC-kernel's hot path.
+ *
+ * Plain C: no HIP, torch or CUDA types cross this boundary.  Host buffers are
+ * caller-owned and copied in; device buffers belong to the context.  Every
+ * function returns SMOC_OK (0) or a negative SMOC_ERR_* code and never
+ * exits the process; smoc_last_error() gives the message of the most recent
+ * failure on the calling thread.  One context per host thread at a time; a
+ * context owns one HIP stream.
+ *
+ * Reference interfaces replaced (ANL-CESAR/SimpleMOC-kernel, Version 4):
+ *   run_kernel(Input*, Source*, Table*)            src/cpu/SimpleMOC-kernel_header.h:81
+ *       and __global__ run_kernel(Input, Source*, Source_Arrays, Table*,
+ *       curandState*, float*, int)                 src/cuda/SimpleMOC-kernel_header.h:66-68
+ *       launched at src/cuda/main.cu:90          -> smoc_run
+ *   attenuate_segment(...)                        src/cpu/SimpleMOC-kernel_header.h:82-84
+ *                                                 -> the sweep kernels behind smoc_run
+ *   initialize_sources / initialize_device_sources
+ *                                                 src/cpu/init.c:26-78, src/cuda/init.cu:69-127
+ *                                                 -> smoc_set_regions (+ smoc_generate_regions)
+ *   the segment draws QSR_id/FAI_id = rand_r()    src/cpu/kernel.c:47-50, src/cuda/kernel.cu:52-60
+ *                                                 -> smoc_set_segments (+ smoc_generate_segments)
+ *   buildExponentialTable / interpolateTable      src/cpu/init.c:81-117, src/cpu/kernel.c:337-361
+ *                                                 -> smoc_build_exp_table, SMOC_EXP_TABLE
+ *   set_default_input + derived region count     src/cpu/init.c:4-24, src/cpu/main.c:18-19
+ *                                                 -> smoc_default_input
+ */
+#ifndef SMOC_H
+#define SMOC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMOC_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------ */
+#define SMOC_OK 0
+#define SMOC_ERR_ARG (-1)         /* invalid argument or unsupported combination */
+#define SMOC_ERR_NODEV (-2)       /* no usable HIP device */
+#define SMOC_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define SMOC_ERR_STATE (-4)       /* call out of order (e.g. run before set_*) */
+#define SMOC_ERR_NOMEM (-5)       /* device or host allocation failed */
+
+/* ---- variants ---------------------------------------------------------- */
+/* Storage of sigT and fine_source in HBM; arithmetic is FP32 in every case. */
+#define SMOC_STORE_F32 0
+#define SMOC_STORE_BF16 1
+#define SMOC_STORE_F16 2
+
+/* ev = 1 - exp(-tau), as in src/cpu/kernel.c:216-223:
+ *   NATIVE    1 - exp(-tau) with the hardware exponential (literal form);
+ *   TABLE     the reference interpolation table (smoc_build_exp_table(0.01,
+ *             10), src/cpu/main.c:33, lookup src/cpu/kernel.c:337-361);
+ *   ACCURATE  native exponential, but for tau < 1 ev, tau - ev and the
+ *             quadratic-source polynomial tau^3 - 3tau^2 + 6tau - 6ev come
+ *             from their Taylor series in tau, which removes the
+ *             cancellation the literal FP32 form suffers for small tau. */
+#define SMOC_EXP_NATIVE 0
+#define SMOC_EXP_TABLE 1
+#define SMOC_EXP_ACCURATE 2
+
+/* fine_flux tally.
+ *   ATOMIC_F32  FP32 device atomics into an FP32 fine_flux (reference CUDA
+ *               src/cuda/kernel.cu:181); order-dependent rounding.
+ *   ATOMIC_F64  FP64 device atomics into an FP64 fine_flux.
+ *   DET_F64     deterministic: each run of consecutive segments of a track
+ *               in one (region, interval) is summed in FP64 registers and
+ *               stored; a segmented reduction then sums the runs of every
+ *               (region, interval) in a fixed order.  Bit-exact run to run.
+ *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums. */
+#define SMOC_TALLY_ATOMIC_F32 0
+#define SMOC_TALLY_ATOMIC_F64 1
+#define SMOC_TALLY_DET_F64 2
+#define SMOC_TALLY_DET_KAHAN 3
+
+typedef struct smoc_ctx smoc_ctx;
+
+typedef struct {
+    int device;      /* HIP device ordinal (reference CUDA -d, src/cuda/io.cu:149-158) */
+    int egroups;     /* G, energy groups (-e)                                  */
+    int regions;     /* R, 3D source regions                                    */
+    int fine_axial;  /* F, fine axial intervals per region (>= 2)               */
+    int storage;     /* SMOC_STORE_*                                            */
+    int exp_mode;    /* SMOC_EXP_*                                              */
+    int tally;       /* SMOC_TALLY_*                                            */
+} smoc_params;
+
+/* Launch configuration.  Zero fields mean "choose": the library's default
+ * or, with autotune = 1, the fastest candidate measured on this problem.
+ * None of these fields changes results except chunk (splitting a track into
+ * chunks re-associates its psi recurrence; see smoc_run). */
+typedef struct {
+    int seg_per_block;   /* -p: segments a workgroup sweeps per work item     */
+    int groups_per_lane; /* 1 or 2 energy groups per lane                     */
+    int waves_per_block; /* wavefronts per workgroup (1..16)                   */
+    int chunk;           /* split tracks longer than this many segments into
+                            chunks joined by an affine psi scan (0 = never)    */
+    int persistent;      /* 1: persistent workgroups pulling work from a queue */
+    int use_graph;       /* 1: capture the run into a hipGraph, replay it      */
+    int autotune;        /* 1: measure candidates on first run, keep the best  */
+} smoc_launch;
+
+typedef struct {
+    double total_ms;   /* whole run, HIP events on the context stream        */
+    double reset_ms;   /* fine_flux reset (and queue counters)              */
+    double sweep_ms;   /* attenuation sweep kernel(s)                        */
+    double reduce_ms;  /* deterministic segmented reduction (DET_* only)     */
+    int64_t segments;  /* segments swept                                     */
+    int64_t work_items;/* (track or chunk) x group-slice items launched      */
+    smoc_launch used;  /* launch configuration actually used                 */
+} smoc_timing;
+
+/* Reference defaults (src/cpu/init.c:4-24, src/cuda/init.cu:30-44). */
+typedef struct {
+    int source_2D_regions;      /* 5000 */
+    int coarse_axial_intervals; /* 27 */
+    int fine_axial_intervals;   /* 5 */
+    int decomp_assemblies_ax;   /* 20 */
+    int64_t segments;           /* 50,000,000 */
+    int egroups;                /* 128 */
+    int seg_per_block;          /* 100 (CUDA -p) */
+    int source_3D_regions;      /* derived: ceil(2D * coarse / decomp) = 6750 */
+} smoc_input;
+
+/* ---- host-side utilities (no GPU needed) ------------------------------- */
+int smoc_abi_version(void);
+const char *smoc_strerror(int code);
+const char *smoc_last_error(void);
+
+/* set_default_input + source_3D_regions (src/cpu/init.c:4-24, main.c:18-19) */
+int smoc_default_input(smoc_input *in);
+/* recompute in->source_3D_regions from the other fields */
+int smoc_derive_regions(smoc_input *in);
+
+/* buildExponentialTable(precision, maxVal) (src/cpu/init.c:81-117).
+ * values receives 2N floats (slope, intercept per interval); capacity is in
+ * floats.  The slope is stored as -exp(-n dx), as the reference does. */
+int smoc_build_exp_table(float precision, float maxVal, float *values, int capacity,
+                         int *N, float *dx, float *maxVal_out);
+/* interpolateTable (src/cpu/kernel.c:337-361) on the host, for checks */
+float smoc_exp_table_lookup(const float *values, float dx, float maxVal, float x);
+
+/* Axial entry point decode used by every kernel: f = min((int)(z*F), F-1),
+ * zin = ((z*F - f) - 0.5) * dz, all in FP32 (dz = 0.1, src/cpu/kernel.c:99). */
+void smoc_decode_z(float z, int F, int *f, float *zin);
+
+/* Seeded, counter-based input generator: element i of each array depends
+ * only on (seed, array, i), so results do not depend on thread count.
+ *   sigT ~ U(0.1, 10) [R][G]; fine_source ~ U(0,1) [R][F][G]. */
+int smoc_generate_regions(uint64_t seed, int G, int R, int F, float *sigT, float *fine_source);
+/*   len ~ U(0, 0.1); region ~ U[0, R); z ~ U(0, 1), for segments [0, S). */
+int smoc_generate_segments(uint64_t seed, int64_t S, int R, float *len, int32_t *region,
+                           float *z);
+/*   psi_in ~ U(0,1) [T][G]. */
+int smoc_generate_psi(uint64_t seed, int64_t T, int G, float *psi_in);
+/* Tracks of L consecutive segments (the last one shorter when S % L != 0).
+ * Returns the track count via *T; track_off needs ceil(S/L)+1 entries. */
+int smoc_tracks_fixed(int64_t S, int L, int64_t *track_off, int64_t *T);
+/* T tracks with lognormal lengths (median, sigma) rounded and clipped to
+ * [lo, hi]; track_off [T+1]; *S receives the total segment count. */
+int smoc_tracks_lognormal(uint64_t seed, int64_t T, double median, double sigma, int lo,
+                          int hi, int64_t *track_off, int64_t *S);
+/* Stable counting sort of the segment stream by (region, fine interval)
+ * key, in place.  Tracks stay defined as contiguous ranges of the stream. */
+int smoc_sort_segments(int64_t S, int R, int F, float *len, int32_t *region, float *z);
+
+/* Fixed-index-order FP64 sums, the result checksums. */
+double smoc_checksum_f64(const double *x, int64_t n);
+double smoc_checksum_f32(const float *x, int64_t n);
+
+/* ---- device path -------------------------------------------------------- */
+int smoc_device_count(int *n);
+/* Device name and compute-unit count for reports. */
+int smoc_device_info(int device, char *name, int name_cap, int *compute_units);
+
+int smoc_create(smoc_ctx **ctx, const smoc_params *p);
+int smoc_destroy(smoc_ctx *ctx);
+
+/* Region data: sigT [R][G], fine_source [R][F][G] (FP32 masters, quantised on
+ * the device to the context's storage type), fine_flux_init [R][F][G] or
+ * NULL for zeros.  May be called again to replace the data. */
+int smoc_set_regions(smoc_ctx *ctx, const float *sigT, const float *fine_source,
+                     const float *fine_flux_init);
+
+/* Segment stream (structure of arrays) and tracks.  Track t sweeps segments
+ * [track_off[t], track_off[t+1]) in order, carrying its angular flux
+ * starting from psi_in[t][0..G).  track_off[0] must be 0 and
+ * track_off[T] == S; region ids must lie in [0, R). */
+int smoc_set_segments(smoc_ctx *ctx, int64_t S, const float *len, const int32_t *region,
+                      const float *z, int64_t T, const int64_t *track_off,
+                      const float *psi_in);
+
+/* One sweep: fine_flux := fine_flux_init + all tallies; psi_out := outgoing
+ * flux of every track.  Inputs are not modified, so repeated runs start from
+ * the same state.  timing may be NULL.  launch may be NULL (defaults). */
+int smoc_run(smoc_ctx *ctx, const smoc_launch *launch, smoc_timing *timing);
+
+/* Copy results to the host: fine_flux [R][F][G] as double, psi_out [T][G].
+ * Either pointer may be NULL. */
+int smoc_get_results(smoc_ctx *ctx, double *fine_flux, float *psi_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,612 @@
+// smoc_api.cpp -- the C ABI of libsmoc (include/smoc.h) on the HIP runtime.
+//
+// A context owns its device buffers and one HIP stream.  Host inputs are
+// copied in once (smoc_set_regions / smoc_set_segments) into padded device
+// layouts; smoc_run only launches kernels on resident data, so its timing is
+// whole-sweep throughput with inputs already in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "smoc.h"
+#include "smoc_internal.h"
+
+namespace {
+
+#define HIPCHK(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            smoc_set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,    \
+                           __LINE__);                                                          \
+            return SMOC_ERR_HIP;                                                               \
+        }                                                                                      \
+    } while (0)
+
+#define LAUNCHCHK(expr)                                                                        \
+    do {                                                                                       \
+        int e_ = (expr);                                                                       \
+        if (e_ != 0) {                                                                         \
+            smoc_set_error("kernel launch %s failed: %s", #expr,                               \
+                           hipGetErrorString((hipError_t)e_));                                 \
+            return SMOC_ERR_HIP;                                                               \
+        }                                                                                      \
+    } while (0)
+
+template <typename T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;  // elements
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t count)
+    {
+        release();
+        if (count == 0)
+            return SMOC_OK;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            smoc_set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                           hipGetErrorString(e));
+            return SMOC_ERR_NOMEM;
+        }
+        n = count;
+        return SMOC_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct smoc_ctx {
+    smoc_params p{};
+    int Gp = 0;  // padded group stride
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int cu_count = 0;
+
+    // region data
+    DevBuf<unsigned char> tuples;
+    DevBuf<float> flux_init;  // [R*F][Gp] or empty (zeros)
+    DevBuf<unsigned char> flux;  // f32 or f64 [R*F][Gp]
+    DevBuf<float> table;
+    float tab_dx = 0.f, tab_max = 0.f;
+    int tab_n = 0;
+    bool regions_set = false;
+
+    // segments
+    int64_t S = 0, T = 0;
+    DevBuf<float> len, z;
+    DevBuf<int32_t> region;
+    DevBuf<int64_t> track_off;
+    DevBuf<float> psi_in, psi_out;  // [T][Gp]
+    bool segments_set = false;
+    bool runs = false;  // sorted-enough stream: sum runs in registers
+
+    // deterministic tally bookkeeping
+    int64_t n_runs = 0;
+    DevBuf<int64_t> track_run_off, fsr_run_off, fsr_run_idx;
+    DevBuf<unsigned char> part;
+
+    DevBuf<int64_t> queue;
+    bool ran = false;
+
+    int64_t n_fsr() const { return (int64_t)p.regions * p.fine_axial; }
+    bool flux_f64() const
+    {
+        return p.tally == SMOC_TALLY_ATOMIC_F64 || p.tally == SMOC_TALLY_DET_F64;
+    }
+    bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
+};
+
+extern "C" {
+
+int smoc_device_count(int *n)
+{
+    if (!n) {
+        smoc_set_error("smoc_device_count: NULL");
+        return SMOC_ERR_ARG;
+    }
+    *n = 0;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        smoc_set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+        return SMOC_ERR_NODEV;
+    }
+    *n = c;
+    return SMOC_OK;
+}
+
+int smoc_device_info(int device, char *name, int name_cap, int *compute_units)
+{
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        smoc_set_error("hipGetDeviceProperties(%d): %s", device, hipGetErrorString(e));
+        return SMOC_ERR_NODEV;
+    }
+    if (name && name_cap > 0)
+        snprintf(name, (size_t)name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+    if (compute_units)
+        *compute_units = prop.multiProcessorCount;
+    return SMOC_OK;
+}
+
+int smoc_create(smoc_ctx **out, const smoc_params *p)
+{
+    if (!out || !p) {
+        smoc_set_error("smoc_create: NULL argument");
+        return SMOC_ERR_ARG;
+    }
+    *out = nullptr;
+    if (p->egroups < 1 || p->egroups > 65536 || p->regions < 1 || p->fine_axial < 2 ||
+        (int64_t)p->regions * p->fine_axial > INT32_MAX / 2) {
+        smoc_set_error("smoc_create: need 1 <= egroups <= 65536, regions >= 1, fine_axial >= 2 "
+                       "(got G=%d R=%d F=%d)",
+                       p->egroups, p->regions, p->fine_axial);
+        return SMOC_ERR_ARG;
+    }
+    if (p->storage < SMOC_STORE_F32 || p->storage > SMOC_STORE_F16 ||
+        p->exp_mode < SMOC_EXP_NATIVE || p->exp_mode > SMOC_EXP_ACCURATE ||
+        p->tally < SMOC_TALLY_ATOMIC_F32 || p->tally > SMOC_TALLY_DET_KAHAN) {
+        smoc_set_error("smoc_create: unknown storage %d / exp_mode %d / tally %d", p->storage,
+                       p->exp_mode, p->tally);
+        return SMOC_ERR_ARG;
+    }
+    int ndev = 0;
+    int rc = smoc_device_count(&ndev);
+    if (rc != SMOC_OK)
+        return rc;
+    if (ndev <= 0 || p->device < 0 || p->device >= ndev) {
+        smoc_set_error("smoc_create: device %d not available (%d HIP devices)", p->device, ndev);
+        return SMOC_ERR_NODEV;
+    }
+    HIPCHK(hipSetDevice(p->device));
+    smoc_ctx *c = new (std::nothrow) smoc_ctx();
+    if (!c) {
+        smoc_set_error("smoc_create: host allocation failed");
+        return SMOC_ERR_NOMEM;
+    }
+    c->p = *p;
+    c->Gp = round_up(p->egroups, 128);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < 4; i++)
+        e = hipEventCreate(&c->ev[i]);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, p->device);
+        c->cu_count = prop.multiProcessorCount;
+    }
+    if (e != hipSuccess) {
+        smoc_set_error("smoc_create: stream/event setup failed: %s", hipGetErrorString(e));
+        smoc_destroy(c);
+        return SMOC_ERR_HIP;
+    }
+    if (p->exp_mode == SMOC_EXP_TABLE) {
+        float vals[SMOC_TABLE_CAP];
+        rc = smoc_build_exp_table(0.01f, 10.0f, vals, SMOC_TABLE_CAP, &c->tab_n, &c->tab_dx,
+                                  &c->tab_max);
+        if (rc == SMOC_OK)
+            rc = c->table.alloc((size_t)2 * c->tab_n);
+        if (rc == SMOC_OK) {
+            e = hipMemcpy(c->table.p, vals, sizeof(float) * 2 * c->tab_n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                smoc_set_error("smoc_create: table upload failed: %s", hipGetErrorString(e));
+                rc = SMOC_ERR_HIP;
+            }
+        }
+        if (rc != SMOC_OK) {
+            smoc_destroy(c);
+            return rc;
+        }
+    }
+    *out = c;
+    return SMOC_OK;
+}
+
+int smoc_destroy(smoc_ctx *c)
+{
+    if (!c)
+        return SMOC_OK;
+    (void)hipSetDevice(c->p.device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    c->tuples.release();
+    c->flux_init.release();
+    c->flux.release();
+    c->table.release();
+    c->len.release();
+    c->z.release();
+    c->region.release();
+    c->track_off.release();
+    c->psi_in.release();
+    c->psi_out.release();
+    c->track_run_off.release();
+    c->fsr_run_off.release();
+    c->fsr_run_idx.release();
+    c->part.release();
+    c->queue.release();
+    for (auto &ev : c->ev)
+        if (ev)
+            (void)hipEventDestroy(ev);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SMOC_OK;
+}
+
+// Copy a [rows][G] host array into a [rows][Gp] device array (padding = 0).
+static int upload_padded(float *dst, const float *src, int64_t rows, int G, int Gp,
+                         hipStream_t s)
+{
+    if (rows == 0)
+        return SMOC_OK;
+    HIPCHK(hipMemsetAsync(dst, 0, sizeof(float) * (size_t)rows * Gp, s));
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * Gp, src, sizeof(float) * G, sizeof(float) * G,
+                            (size_t)rows, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SMOC_OK;
+}
+
+int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
+                     const float *fine_flux_init)
+{
+    if (!c || !sigT || !fine_source) {
+        smoc_set_error("smoc_set_regions: NULL argument");
+        return SMOC_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    const int G = c->p.egroups, R = c->p.regions, F = c->p.fine_axial;
+    const int64_t nsig = (int64_t)R * G, nsrc = (int64_t)R * F * G;
+    for (int64_t i = 0; i < nsig; i++)
+        if (!std::isfinite(sigT[i]) || !(sigT[i] > 0.f)) {
+            smoc_set_error("smoc_set_regions: sigT[%lld] = %g must be finite and > 0",
+                           (long long)i, (double)sigT[i]);
+            return SMOC_ERR_ARG;
+        }
+    for (int64_t i = 0; i < nsrc; i++)
+        if (!std::isfinite(fine_source[i])) {
+            smoc_set_error("smoc_set_regions: fine_source[%lld] is not finite", (long long)i);
+            return SMOC_ERR_ARG;
+        }
+    const int64_t nrow = c->n_fsr();
+    int rc = c->tuples.alloc((size_t)nrow * c->Gp * smoc_tuple_bytes(c->p.storage));
+    if (rc == SMOC_OK)
+        rc = c->flux.alloc((size_t)nrow * c->Gp * (c->flux_f64() ? 8 : 4));
+    if (rc != SMOC_OK)
+        return rc;
+    // FP32 masters are staged on the device, packed (and quantised) there
+    DevBuf<float> dsig, dsrc;
+    rc = dsig.alloc((size_t)nsig);
+    if (rc == SMOC_OK)
+        rc = dsrc.alloc((size_t)nsrc);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpyAsync(dsig.p, sigT, sizeof(float) * nsig, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dsrc.p, fine_source, sizeof(float) * nsrc, hipMemcpyHostToDevice,
+                          c->stream));
+    smoc_pack_args pa{dsig.p, dsrc.p, c->tuples.p, G, c->Gp, R, F};
+    LAUNCHCHK(smoc_launch_pack(&pa, c->p.storage, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (fine_flux_init) {
+        rc = c->flux_init.alloc((size_t)nrow * c->Gp);
+        if (rc == SMOC_OK)
+            rc = upload_padded(c->flux_init.p, fine_flux_init, nrow, G, c->Gp, c->stream);
+        if (rc != SMOC_OK)
+            return rc;
+    } else {
+        c->flux_init.release();
+    }
+    c->regions_set = true;
+    c->ran = false;
+    return SMOC_OK;
+}
+
+// Run bookkeeping of the deterministic tally: runs are numbered track by
+// track in stream order; a run is a maximal stretch of one track's segments
+// with the same (region, interval) when c->runs, else a single segment.
+// fsr_run_idx lists, for every (region, interval), its runs in ascending id.
+static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float *z, int64_t T,
+                      const int64_t *track_off)
+{
+    const int F = c->p.fine_axial;
+    const int64_t K = c->n_fsr();
+    std::vector<int32_t> run_key;
+    std::vector<int64_t> tro((size_t)T + 1);
+    try {
+        run_key.reserve(c->runs ? (size_t)T + 1024 : (size_t)S);
+    } catch (...) {
+        smoc_set_error("smoc_set_segments: host allocation for run bookkeeping failed");
+        return SMOC_ERR_NOMEM;
+    }
+    for (int64_t t = 0; t < T; t++) {
+        tro[t] = (int64_t)run_key.size();
+        int32_t prev = -1;
+        for (int64_t k = track_off[t]; k < track_off[t + 1]; k++) {
+            int f;
+            float zin;
+            smoc_decode_z(z[k], F, &f, &zin);
+            const int32_t key = region[k] * F + f;
+            if (!c->runs || key != prev)
+                run_key.push_back(key);
+            prev = key;
+        }
+    }
+    tro[T] = (int64_t)run_key.size();
+    const int64_t n_runs = tro[T];
+    std::vector<int64_t> fro((size_t)K + 1, 0), idx((size_t)std::max<int64_t>(n_runs, 1));
+    for (int64_t r = 0; r < n_runs; r++)
+        fro[run_key[r] + 1]++;
+    for (int64_t k = 0; k < K; k++)
+        fro[k + 1] += fro[k];
+    {
+        std::vector<int64_t> pos(fro.begin(), fro.end() - 1);
+        for (int64_t r = 0; r < n_runs; r++)
+            idx[pos[run_key[r]]++] = r;
+    }
+    int rc = c->track_run_off.alloc((size_t)T + 1);
+    if (rc == SMOC_OK)
+        rc = c->fsr_run_off.alloc((size_t)K + 1);
+    if (rc == SMOC_OK)
+        rc = c->fsr_run_idx.alloc((size_t)std::max<int64_t>(n_runs, 1));
+    if (rc == SMOC_OK)
+        rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * 8);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpy(c->track_run_off.p, tro.data(), sizeof(int64_t) * (T + 1),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fsr_run_off.p, fro.data(), sizeof(int64_t) * (K + 1),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->fsr_run_idx.p, idx.data(),
+                     sizeof(int64_t) * std::max<int64_t>(n_runs, 1), hipMemcpyHostToDevice));
+    c->n_runs = n_runs;
+    return SMOC_OK;
+}
+
+int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *region,
+                      const float *z, int64_t T, const int64_t *track_off, const float *psi_in)
+{
+    if (!c || S < 0 || T < 0 || !track_off || (S > 0 && (!len || !region || !z)) ||
+        (T > 0 && !psi_in)) {
+        smoc_set_error("smoc_set_segments: bad argument");
+        return SMOC_ERR_ARG;
+    }
+    if (S > INT32_MAX) {
+        smoc_set_error("smoc_set_segments: %lld segments exceed 2^31-1", (long long)S);
+        return SMOC_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    const int R = c->p.regions, G = c->p.egroups;
+    // every index a kernel derives from the stream is checked here
+    if (track_off[0] != 0 || track_off[T] != S) {
+        smoc_set_error("smoc_set_segments: track_off[0] = %lld and track_off[T] = %lld must be "
+                       "0 and S = %lld",
+                       (long long)track_off[0], (long long)track_off[T], (long long)S);
+        return SMOC_ERR_ARG;
+    }
+    for (int64_t t = 0; t < T; t++)
+        if (track_off[t + 1] < track_off[t]) {
+            smoc_set_error("smoc_set_segments: track_off not monotone at %lld", (long long)t);
+            return SMOC_ERR_ARG;
+        }
+    int64_t same = 0;
+    for (int64_t k = 0; k < S; k++) {
+        if (region[k] < 0 || region[k] >= R) {
+            smoc_set_error("smoc_set_segments: region[%lld] = %d outside [0, %d)", (long long)k,
+                           region[k], R);
+            return SMOC_ERR_ARG;
+        }
+        if (!std::isfinite(len[k]) || !std::isfinite(z[k]) || len[k] < 0.f) {
+            smoc_set_error("smoc_set_segments: segment %lld has len %g, z %g", (long long)k,
+                           (double)len[k], (double)z[k]);
+            return SMOC_ERR_ARG;
+        }
+        if (k > 0 && region[k] == region[k - 1])
+            same++;
+    }
+    for (int64_t i = 0; i < T * G; i++)
+        if (!std::isfinite(psi_in[i])) {
+            smoc_set_error("smoc_set_segments: psi_in[%lld] is not finite", (long long)i);
+            return SMOC_ERR_ARG;
+        }
+    // A stream sorted by region keeps most neighbours in the same region;
+    // then runs are summed in registers.
+    c->runs = S > 1 && 2 * same >= S;
+
+    int rc = c->len.alloc((size_t)S);
+    if (rc == SMOC_OK)
+        rc = c->z.alloc((size_t)S);
+    if (rc == SMOC_OK)
+        rc = c->region.alloc((size_t)S);
+    if (rc == SMOC_OK)
+        rc = c->track_off.alloc((size_t)T + 1);
+    if (rc == SMOC_OK)
+        rc = c->psi_in.alloc((size_t)T * c->Gp);
+    if (rc == SMOC_OK)
+        rc = c->psi_out.alloc((size_t)T * c->Gp);
+    if (rc != SMOC_OK)
+        return rc;
+    if (S > 0) {
+        HIPCHK(hipMemcpy(c->len.p, len, sizeof(float) * S, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->z.p, z, sizeof(float) * S, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->region.p, region, sizeof(int32_t) * S, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c->track_off.p, track_off, sizeof(int64_t) * (T + 1),
+                     hipMemcpyHostToDevice));
+    rc = upload_padded(c->psi_in.p, psi_in, T, G, c->Gp, c->stream);
+    if (rc != SMOC_OK)
+        return rc;
+    c->S = S;
+    c->T = T;
+    c->n_runs = 0;
+    if (c->det()) {
+        rc = build_runs(c, S, region, z, T, track_off);
+        if (rc != SMOC_OK)
+            return rc;
+    }
+    c->segments_set = true;
+    c->ran = false;
+    return SMOC_OK;
+}
+
+static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l)
+{
+    smoc_launch d{};
+    if (in)
+        d = *in;
+    if (d.groups_per_lane != 1 && d.groups_per_lane != 2)
+        d.groups_per_lane = c->p.egroups > 64 ? 2 : 1;
+    if (d.waves_per_block < 1 || d.waves_per_block > 16)
+        d.waves_per_block = 4;
+    if (d.seg_per_block <= 0)
+        d.seg_per_block = 100;
+    *l = d;
+}
+
+int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
+{
+    if (!c) {
+        smoc_set_error("smoc_run: NULL context");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->regions_set || !c->segments_set) {
+        smoc_set_error("smoc_run: call smoc_set_regions and smoc_set_segments first");
+        return SMOC_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    smoc_launch l;
+    fill_launch(c, launch, &l);
+    const int G = c->p.egroups;
+    const int gpl = l.groups_per_lane;
+    const int slices = (G + 64 * gpl - 1) / (64 * gpl);
+    const int64_t nrow = c->n_fsr();
+
+    smoc_sweep_args a{};
+    a.tuples = c->tuples.p;
+    a.len = c->len.p;
+    a.region = c->region.p;
+    a.z = c->z.p;
+    a.track_off = c->track_off.p;
+    a.psi_in = c->psi_in.p;
+    a.psi_out = c->psi_out.p;
+    a.flux = c->det() ? (void *)c->part.p : (void *)c->flux.p;
+    a.track_run_off = c->track_run_off.p;
+    a.table = c->table.p;
+    a.items = nullptr;
+    a.chunk_psi = nullptr;
+    a.n_items = c->T;
+    a.queue = nullptr;
+    a.tab_dx = c->tab_dx;
+    a.tab_max = c->tab_max;
+    a.tab_n = c->tab_n;
+    a.G = G;
+    a.Gp = c->Gp;
+    a.F = c->p.fine_axial;
+    a.slices = slices;
+    a.runs = c->runs ? 1 : 0;
+
+    const int64_t n_work = c->T * slices;
+    int64_t grid64 = (n_work + l.waves_per_block - 1) / l.waves_per_block;
+    if (l.persistent) {
+        if (!c->queue.p) {
+            int rc = c->queue.alloc(1);
+            if (rc != SMOC_OK)
+                return rc;
+        }
+        a.queue = c->queue.p;
+        grid64 = std::min<int64_t>(grid64, (int64_t)c->cu_count * (32 / l.waves_per_block > 0 ? 32 / l.waves_per_block : 1));
+    }
+    if (grid64 > INT32_MAX) {
+        smoc_set_error("smoc_run: %lld workgroups exceed the grid limit", (long long)grid64);
+        return SMOC_ERR_ARG;
+    }
+
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    if (!c->det())
+        LAUNCHCHK(smoc_launch_reset(c->flux.p, c->flux_init.p, nrow * c->Gp, c->flux_f64() ? 1 : 0,
+                                    c->stream));
+    if (a.queue)
+        LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally,
+                                l.waves_per_block, (int)grid64, c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    if (c->det()) {
+        smoc_reduce_args r{};
+        r.part = c->part.p;
+        r.fsr_run_off = c->fsr_run_off.p;
+        r.fsr_run_idx = c->fsr_run_idx.p;
+        r.init = c->flux_init.p;
+        r.flux = c->flux.p;
+        r.n_fsr = nrow;
+        r.Gp = c->Gp;
+        r.slices = slices;
+        LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, 4, c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    c->ran = true;
+    if (timing) {
+        float ms[3] = {0.f, 0.f, 0.f};
+        HIPCHK(hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
+        HIPCHK(hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
+        HIPCHK(hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]));
+        timing->reset_ms = ms[0];
+        timing->sweep_ms = ms[1];
+        timing->reduce_ms = ms[2];
+        timing->total_ms = (double)ms[0] + ms[1] + ms[2];
+        timing->segments = c->S;
+        timing->work_items = n_work;
+        timing->used = l;
+    }
+    return SMOC_OK;
+}
+
+int smoc_get_results(smoc_ctx *c, double *fine_flux, float *psi_out)
+{
+    if (!c) {
+        smoc_set_error("smoc_get_results: NULL context");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->ran) {
+        smoc_set_error("smoc_get_results: no completed smoc_run since the last set_*");
+        return SMOC_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    const int G = c->p.egroups, Gp = c->Gp;
+    const int64_t nrow = c->n_fsr();
+    if (fine_flux) {
+        if (c->flux_f64()) {
+            HIPCHK(hipMemcpy2D(fine_flux, sizeof(double) * G, c->flux.p, sizeof(double) * Gp,
+                               sizeof(double) * G, (size_t)nrow, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp((size_t)nrow * G);
+            HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * G, c->flux.p, sizeof(float) * Gp,
+                               sizeof(float) * G, (size_t)nrow, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < tmp.size(); i++)
+                fine_flux[i] = tmp[i];
+        }
+    }
+    if (psi_out && c->T > 0)
+        HIPCHK(hipMemcpy2D(psi_out, sizeof(float) * G, c->psi_out.p, sizeof(float) * Gp,
+                           sizeof(float) * G, (size_t)c->T, hipMemcpyDeviceToHost));
+    return SMOC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+/*
+ * smoc_internal.h -- declarations shared by the host C code, the C-ABI
+ * implementation and the HIP kernels of libsmoc.  Not installed.
+ */
+#ifndef SMOC_INTERNAL_H
+#define SMOC_INTERNAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Placeholder geometry constants of src/cpu/kernel.c:99-103 (ds and zin are
+ * per-segment inputs in this library). */
+#define SMOC_DZ 0.1f
+#define SMOC_WEIGHT 0.5f
+#define SMOC_MU 0.9f
+#define SMOC_MU2 0.3f
+
+/* exp table capacity in floats (the reference table has N = 353 intervals) */
+#define SMOC_TABLE_CAP 1024
+
+void smoc_set_error(const char *fmt, ...);
+
+/* Arguments of the attenuation sweep.  One work item is (track or chunk,
+ * group slice); a slice covers 64 * groups_per_lane consecutive groups. */
+typedef struct {
+    const void *tuples;          /* packed region data [R*F][Gp] (see pack)   */
+    const float *len;            /* [S] */
+    const int32_t *region;       /* [S] */
+    const float *z;              /* [S] */
+    const int64_t *track_off;    /* [T+1] */
+    const float *psi_in;         /* [T][Gp] */
+    float *psi_out;              /* [T][Gp] */
+    void *flux;                  /* ATOMIC_*: fine_flux [R*F][Gp]; DET_*: run partials [runs][Gp] */
+    const int64_t *track_run_off;/* DET_*: first run id of each track [T+1]  */
+    const float *table;          /* exp table, 2N floats (EXP_TABLE)          */
+    /* chunked sweeps (tracks split into chunks) */
+    const int64_t *items;        /* NULL: item i = track i; else [n][3]: track, begin, end */
+    const float *chunk_psi;      /* [n_items][Gp] incoming psi of each chunk, or NULL */
+    int64_t n_items;             /* tracks or chunks */
+    int64_t *queue;              /* persistent mode: work counter, or NULL    */
+    float tab_dx, tab_max;
+    int tab_n;
+    int G, Gp, F, slices;        /* Gp: padded row stride (multiple of 128) */
+    int runs;                    /* 1: sum runs of equal (region, interval) in
+                                    registers; 0: every segment is a run */
+} smoc_sweep_args;
+
+typedef struct {
+    const float *sigT;        /* [R][G] FP32 master */
+    const float *fine_source; /* [R][F][G] FP32 master */
+    void *tuples;             /* out: [R*F][Gp] */
+    int G, Gp, R, F;
+} smoc_pack_args;
+
+typedef struct {
+    const void *part;            /* run partials */
+    const int64_t *fsr_run_off;  /* [R*F+1] */
+    const int64_t *fsr_run_idx;  /* [n_runs] run ids grouped by (region, interval) */
+    const float *init;           /* fine_flux init [R*F][Gp] or NULL */
+    void *flux;                  /* out fine_flux [R*F][Gp]: f64 (DET_F64) / f32 (DET_KAHAN) */
+    int64_t n_fsr;
+    int Gp, slices;
+} smoc_reduce_args;
+
+/* kernel launchers (smoc_kernels.hip); return 0 or a hipError_t value */
+int smoc_launch_pack(const smoc_pack_args *a, int storage, void *stream);
+int smoc_launch_reset(void *flux, const float *init, int64_t n, int f64, void *stream);
+int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl, int exp_mode, int tally,
+                      int waves_per_block, int grid_blocks, void *stream);
+int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_per_block,
+                       void *stream);
+int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
+/* bytes per packed tuple for a storage type */
+int smoc_tuple_bytes(int storage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

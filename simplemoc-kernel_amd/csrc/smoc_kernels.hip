@@ -1,0 +1,746 @@
+// smoc_kernels.hip -- gfx950 kernels of the segment-attenuation sweep.
+//
+// The sweep restates the arithmetic of the reference attenuate_segment
+// (src/cpu/kernel.c:108-332; CUDA twin src/cuda/kernel.cu:90-192) for one
+// (segment, energy group) pair per lane and step, with the segment stream
+// walked in track order so the angular flux psi is carried in registers.
+//
+// Mapping (CDNA4, wave64):
+//   * one wavefront sweeps one work item = (track or chunk of a track, group
+//     slice); a slice is 64 * GPL consecutive groups, lane l owning groups
+//     slice*64*GPL + j*64 + l for j < GPL;
+//   * the segment stream (len, region, z: structure of arrays) is read with
+//     wave-uniform addresses, i.e. scalar loads, one segment ahead of the
+//     arithmetic, together with the vector gather of the next segment's
+//     packed region tuple;
+//   * region data is packed once per smoc_set_regions into one tuple per
+//     (region, fine interval, group): FP32 storage keeps (sigT, c0, c1, c2)
+//     of the source fit, 16-bit storage keeps (sigT, y[f-1], y[f], y[f+1])
+//     and fits in registers;
+//   * consecutive segments of a track in the same (region, interval) form a
+//     run whose tally is summed in registers and flushed once (an atomic,
+//     or a plain store of the run partial in the deterministic modes);
+//   * block -> work mapping is XCD-aware: the 8 XCDs each get a contiguous
+//     range of work items (hardware dispatches blocks round-robin over XCDs).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "smoc.h"
+#include "smoc_internal.h"
+
+namespace {
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kDz = SMOC_DZ;
+// divisions by the fit's constant denominators (kernel.c:129,155,183-184)
+constexpr float kInvDz = 1.0f / kDz;
+constexpr float kInv2Dz = 1.0f / (2.f * kDz);
+constexpr float kInv2Dz2 = 1.0f / (2.f * kDz * kDz);
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// ---- storage formats -------------------------------------------------------
+template <int STORE> struct Tup;
+template <> struct Tup<SMOC_STORE_F32> {
+    using raw = float4;
+    static __device__ __forceinline__ raw one() { return make_float4(1.f, 0.f, 0.f, 0.f); }
+};
+template <> struct Tup<SMOC_STORE_BF16> {
+    using raw = uint2;
+    static __device__ __forceinline__ raw one() { return make_uint2(0x3F80u, 0u); }
+};
+template <> struct Tup<SMOC_STORE_F16> {
+    using raw = uint2;
+    static __device__ __forceinline__ raw one() { return make_uint2(0x3C00u, 0u); }
+};
+
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ float h_to_f(uint32_t h16)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h16);
+}
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint32_t f_to_bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ uint32_t f_to_h(float x)
+{
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+template <int STORE>
+__device__ __forceinline__ float4 unpack(typename Tup<STORE>::raw r)
+{
+    if constexpr (STORE == SMOC_STORE_F32) {
+        return r;
+    } else if constexpr (STORE == SMOC_STORE_BF16) {
+        return make_float4(bf_lo(r.x), bf_hi(r.x), bf_lo(r.y), bf_hi(r.y));
+    } else {
+        return make_float4(h_to_f(r.x & 0xFFFFu), h_to_f(r.x >> 16), h_to_f(r.y & 0xFFFFu),
+                           h_to_f(r.y >> 16));
+    }
+}
+
+// ---- segment geometry -----------------------------------------------------
+struct RawSeg {
+    float len, z;
+    int region;
+};
+
+struct Seg {
+    float ds;   // segment length
+    float zin;  // axial offset inside the fine interval
+    int f;      // fine axial interval
+    int row;    // region * F + f, -1 past the end of the item
+};
+
+__device__ __forceinline__ RawSeg load_raw(const float *__restrict__ len,
+                                           const int32_t *__restrict__ region,
+                                           const float *__restrict__ z, int64_t k)
+{
+    return RawSeg{len[k], z[k], region[k]};
+}
+
+// Same FP32 operations as smoc_decode_z on the host (no contraction).
+__device__ __forceinline__ Seg decode(const RawSeg r, int F)
+{
+#pragma clang fp contract(off)
+    Seg s;
+    s.ds = r.len;
+    const float zf = r.z * (float)F;
+    int f = (int)zf;
+    f = f > F - 1 ? F - 1 : f;
+    f = f < 0 ? 0 : f;
+    s.f = f;
+    s.zin = ((zf - (float)f) - 0.5f) * kDz;
+    s.row = r.region * F + f;
+    return s;
+}
+
+__device__ __forceinline__ Seg past_end() { return Seg{0.f, 0.f, 0, -1}; }
+
+// Source fit q0, q1, q2 at zin (kernel.c:111-191).
+template <int STORE>
+__device__ __forceinline__ void fit(const float4 t, int f, int F, float zin, float zin2,
+                                    float &q0, float &q1, float &q2)
+{
+    float c0, c1, c2;
+    if constexpr (STORE == SMOC_STORE_F32) {
+        c0 = t.y;  // coefficients were fitted by the pack kernel
+        c1 = t.z;
+        c2 = t.w;
+    } else {
+        c0 = t.z;
+        if (f == 0) {
+            c1 = (t.w - t.z) * kInvDz;
+            c2 = 0.f;
+        } else if (f == F - 1) {
+            c1 = (t.z - t.y) * kInvDz;
+            c2 = 0.f;
+        } else {
+            c1 = (t.y - t.w) * kInv2Dz;
+            c2 = ((t.y - 2.f * t.z) + t.w) * kInv2Dz2;
+        }
+    }
+    q0 = c0 + c1 * zin + c2 * zin2;
+    q1 = c1 + 2.f * c2 * zin;
+    q2 = c2;
+}
+
+struct ExpTab {
+    const float *lds;  // 2N floats in LDS
+    float dx, hdx, tmax;
+    int n;
+};
+
+// ev = 1 - e^-tau and the two cancellation-prone combinations of the flux
+// integral, tme = tau - ev and poly = tau(tau(tau-3)+6) - 6ev (kernel.c:235-
+// 252, 301).
+struct EvTerms {
+    float ev, one_m_ev, tme, poly;
+};
+
+// Taylor series for 0 <= tau < 1: r4 = sum_{k>=0} (-tau)^k / (k+4)!,
+// r = 1/2 - tau/6 + tau^2 r4; then ev = tau - tau^2 r, tme = tau^2 r and
+// poly = 6 tau^4 r4 without subtracting nearly equal numbers.  Degree 7 in
+// r4 truncates at tau^8/12! * 4! < 5e-8 relative for tau < 1.
+__device__ __forceinline__ EvTerms ev_series(float tau)
+{
+    float r4 = -1.f / 39916800.f;
+    r4 = r4 * tau + 1.f / 3628800.f;
+    r4 = r4 * tau - 1.f / 362880.f;
+    r4 = r4 * tau + 1.f / 40320.f;
+    r4 = r4 * tau - 1.f / 5040.f;
+    r4 = r4 * tau + 1.f / 720.f;
+    r4 = r4 * tau - 1.f / 120.f;
+    r4 = r4 * tau + 1.f / 24.f;
+    const float t2 = tau * tau;
+    const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
+    const float tme = t2 * r;
+    const float ev = tau - tme;
+    return EvTerms{ev, 1.f - ev, tme, 6.f * (t2 * t2) * r4};
+}
+
+__device__ __forceinline__ EvTerms ev_direct(float tau, float ev, float one_m_ev)
+{
+    return EvTerms{ev, one_m_ev, tau - ev, tau * (tau * (tau - 3.f) + 6.f) - 6.f * ev};
+}
+
+template <int EXPM>
+__device__ __forceinline__ EvTerms ev_terms(float tau, const ExpTab &tb)
+{
+    if constexpr (EXPM == SMOC_EXP_TABLE) {
+        // interpolateTable (kernel.c:337-361), interval from the IEEE quotient
+        float ev;
+        if (tau > tb.tmax) {
+            ev = 1.f;
+        } else {
+            int iv = (int)(tau / tb.dx + tb.hdx);
+            iv = iv < 0 ? 0 : (iv > tb.n - 1 ? tb.n - 1 : iv);
+            const float2 sl = reinterpret_cast<const float2 *>(tb.lds)[iv];
+            ev = sl.x * tau + sl.y;
+        }
+        return ev_direct(tau, ev, 1.f - ev);
+    } else if constexpr (EXPM == SMOC_EXP_NATIVE) {
+        const float e = __builtin_amdgcn_exp2f(-tau * kLog2e);  // e^-tau, tau >= 0
+        return ev_direct(tau, 1.f - e, e);
+    } else {
+        // series for tau < 1; the direct forms only where a lane needs them
+        // (wave-uniform test, so waves with tau < 1 everywhere skip them)
+        EvTerms o = ev_series(tau);
+        if (__any(tau >= 1.f)) {
+            const float e = __builtin_amdgcn_exp2f(-tau * kLog2e);
+            const EvTerms d = ev_direct(tau, 1.f - e, e);
+            if (tau >= 1.f)
+                o = d;
+        }
+        return o;
+    }
+}
+
+// Attenuation of one (segment, group): kernel.c:200-332.  Returns the tally
+// (weight * flux_integral) and updates psi.
+template <int EXPM>
+__device__ __forceinline__ float attenuate(float sig, float q0, float q1, float q2, float ds,
+                                           float &psi, const ExpTab &tb)
+{
+    const float tau = sig * ds;
+    const EvTerms x = ev_terms<EXPM>(tau, tb);
+    const float ev = x.ev;
+    const float i1 = rcp(sig);
+    const float i2 = i1 * i1;
+    const float i3 = i2 * i1;
+    const float reuse = tau * (tau - 2.f) + 2.f * ev * i3;
+    const float flux_integral = (q0 * tau + (sig * psi - q0) * ev) * i2 + q1 * SMOC_MU * reuse +
+                                q2 * SMOC_MU2 * x.poly * (i2 * i2 * (1.f / 3.f));
+    psi = q0 * ev * i1 + q1 * SMOC_MU * x.tme * i2 + q2 * SMOC_MU2 * reuse + psi * x.one_m_ev;
+    return SMOC_WEIGHT * flux_integral;
+}
+
+// ---- tally accumulators ------------------------------------------------------
+template <int TALLY> struct Acc;
+template <> struct Acc<SMOC_TALLY_ATOMIC_F32> {
+    float s = 0.f;
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
+    {
+        unsafeAtomicAdd(static_cast<float *>(flux) + row * G + g, s);
+        s = 0.f;
+    }
+};
+template <> struct Acc<SMOC_TALLY_ATOMIC_F64> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(float x) { s += (double)x; }
+    __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
+    {
+        unsafeAtomicAdd(static_cast<double *>(flux) + row * G + g, s);
+        s = 0.0;
+    }
+};
+template <> struct Acc<SMOC_TALLY_DET_F64> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(float x) { s += (double)x; }
+    __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
+    {
+        static_cast<double *>(part)[run * G + g] = s;
+        s = 0.0;
+    }
+};
+template <> struct Acc<SMOC_TALLY_DET_KAHAN> {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x)
+    {
+#pragma clang fp contract(off)
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
+    {
+        static_cast<float2 *>(part)[run * G + g] = make_float2(s, c);
+        s = 0.f;
+        c = 0.f;
+    }
+};
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b & 7u, i = b >> 3, per = nb >> 3, rem = nb & 7u;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+// ---- the sweep ------------------------------------------------------------------
+__device__ __forceinline__ int64_t uniform64(int64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// The segment-stream pointers reach sweep_item as separate restrict kernel
+// parameters so that their wave-uniform loads compile to scalar loads.
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+__device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
+                                           const float *__restrict__ len,
+                                           const int32_t *__restrict__ region,
+                                           const float *__restrict__ z,
+                                           const int64_t *__restrict__ track_off,
+                                           const int64_t *__restrict__ items, int64_t item,
+                                           int lane, const ExpTab &tb)
+{
+    using raw_t = typename Tup<STORE>::raw;
+    const int F = a.F;
+    const int64_t it = a.slices == 1 ? item : item / a.slices;
+    const int sl = (int)(item - it * a.slices);
+
+    int64_t t, kb, ke;
+    if (items) {
+        t = uniform64(items[3 * it]);
+        kb = uniform64(items[3 * it + 1]);
+        ke = uniform64(items[3 * it + 2]);
+    } else {
+        t = it;
+        kb = uniform64(track_off[t]);
+        ke = uniform64(track_off[t + 1]);
+    }
+
+    // Device rows are padded to Gp >= slices * 64 * GPL groups, so every lane
+    // addresses a valid element; lanes past G compute on padding and write
+    // only padding.
+    const int Gp = a.Gp;
+    int g[GPL];
+    float psi[GPL];
+    Acc<TALLY> acc[GPL];
+    const float *pin = a.chunk_psi ? a.chunk_psi + it * Gp : a.psi_in + t * Gp;
+#pragma unroll
+    for (int j = 0; j < GPL; j++) {
+        g[j] = sl * 64 * GPL + j * 64 + lane;
+        psi[j] = pin[g[j]];
+    }
+    int64_t run = 0;
+    if constexpr (TALLY == SMOC_TALLY_DET_F64 || TALLY == SMOC_TALLY_DET_KAHAN)
+        run = uniform64(a.track_run_off[t]);
+
+    const raw_t *__restrict__ tup = static_cast<const raw_t *>(a.tuples);
+    // Software pipeline over the item's segments: while segment k is
+    // computed from tuple buffer X (or Y), the tuples of k+1 are in flight
+    // into the other buffer and the stream entries of k+2 into scalar
+    // registers.  Two explicit buffers avoid register copies that would force
+    // a wait on the freshly issued gathers.
+    // Gathers and (without RUNS) flushes are unconditional so that the
+    // compiler's memory-counter waits stay exact: a load issued on only one
+    // side of a branch makes every later wait assume the shorter path and
+    // stall on the loads just issued.  Past the end, row 0 is read and unused.
+    auto gather = [&](raw_t (&buf)[GPL], const Seg &sg) {
+        const int64_t row = sg.row >= 0 ? sg.row : 0;
+#pragma unroll
+        for (int j = 0; j < GPL; j++)
+            buf[j] = tup[row * Gp + g[j]];
+    };
+    auto raw_at = [&](int64_t k) {
+        return k < ke ? load_raw(len, region, z, k) : RawSeg{0.f, 0.f, -1};
+    };
+    auto step = [&](const raw_t (&buf)[GPL], const Seg &cur, const Seg &nxt) {
+        const float zin2 = cur.zin * cur.zin;
+#pragma unroll
+        for (int j = 0; j < GPL; j++) {
+            const float4 r = unpack<STORE>(buf[j]);
+            float q0, q1, q2;
+            fit<STORE>(r, cur.f, F, cur.zin, zin2, q0, q1, q2);
+            acc[j].add(attenuate<EXPM>(r.x, q0, q1, q2, cur.ds, psi[j], tb));
+        }
+        // RUNS: the stream is sorted enough that a track stays in one
+        // (region, interval) for several segments; sum the run in registers
+        // and flush when it ends.  Otherwise every segment is its own run.
+        if (!RUNS || nxt.row != cur.row) {
+#pragma unroll
+            for (int j = 0; j < GPL; j++)
+                acc[j].flush(a.flux, cur.row, run, Gp, g[j]);
+            ++run;
+        }
+    };
+    auto dec = [&](const RawSeg &r) { return r.region >= 0 ? decode(r, F) : past_end(); };
+
+    if (kb < ke) {
+        raw_t X[GPL], Y[GPL];
+#pragma unroll
+        for (int j = 0; j < GPL; j++)
+            X[j] = Y[j] = Tup<STORE>::one();
+        Seg s0 = dec(raw_at(kb));
+        gather(X, s0);
+        Seg s1 = dec(raw_at(kb + 1));
+        for (int64_t k = kb;;) {
+            gather(Y, s1);
+            const RawSeg r2 = raw_at(k + 2);
+            step(X, s0, s1);
+            const Seg s2 = dec(r2);
+            if (++k >= ke)
+                break;
+            gather(X, s2);
+            const RawSeg r3 = raw_at(k + 2);
+            step(Y, s1, s2);
+            const Seg s3 = dec(r3);
+            if (++k >= ke)
+                break;
+            s0 = s2;
+            s1 = s3;
+        }
+    }
+    // chunks other than a track's last carry no psi_out
+    if (!items || ke == uniform64(track_off[t + 1])) {
+        float *pout = a.psi_out + t * Gp;
+#pragma unroll
+        for (int j = 0; j < GPL; j++)
+            pout[g[j]] = psi[j];
+    }
+}
+
+template <int EXPM>
+__device__ __forceinline__ ExpTab stage_table(const smoc_sweep_args &a, float *s_tab)
+{
+    ExpTab tb{nullptr, 0.f, 0.f, 0.f, 0};
+    if constexpr (EXPM == SMOC_EXP_TABLE) {
+        for (int i = threadIdx.x; i < 2 * a.tab_n; i += blockDim.x)
+            s_tab[i] = a.table[i];
+        __syncthreads();
+        tb.lds = s_tab;
+        tb.dx = a.tab_dx;
+        tb.hdx = 0.5f * a.tab_dx;
+        tb.tmax = a.tab_max;
+        tb.n = a.tab_n;
+    }
+    return tb;
+}
+
+// One wave per work item, XCD-aware block order.
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+__global__ void __launch_bounds__(1024)
+    k_sweep(smoc_sweep_args a, const float *__restrict__ len, const int32_t *__restrict__ region,
+            const float *__restrict__ z, const int64_t *__restrict__ track_off,
+            const int64_t *__restrict__ items)
+{
+    __shared__ float s_tab[EXPM == SMOC_EXP_TABLE ? SMOC_TABLE_CAP : 1];
+    const ExpTab tb = stage_table<EXPM>(a, s_tab);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int64_t item = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * wpb + wave;
+    if (item >= a.n_items * a.slices)
+        return;
+    sweep_item<STORE, GPL, EXPM, TALLY, RUNS>(a, len, region, z, track_off, items, item, lane, tb);
+}
+
+// Persistent variant: each wave pulls work items from a device counter until
+// none remain; every wave exits after at most n_work + 1 pulls.
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+__global__ void __launch_bounds__(1024)
+    k_sweep_persistent(smoc_sweep_args a, const float *__restrict__ len,
+                       const int32_t *__restrict__ region, const float *__restrict__ z,
+                       const int64_t *__restrict__ track_off, const int64_t *__restrict__ items)
+{
+    __shared__ float s_tab[EXPM == SMOC_EXP_TABLE ? SMOC_TABLE_CAP : 1];
+    const ExpTab tb = stage_table<EXPM>(a, s_tab);
+    const int lane = threadIdx.x & 63;
+    const int64_t n_work = a.n_items * a.slices;
+    for (;;) {
+        unsigned long long got = 0;
+        if (lane == 0)
+            got = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), 1ull);
+        const int64_t item = uniform64((int64_t)__shfl(got, 0));
+        if (item >= n_work)
+            break;
+        sweep_item<STORE, GPL, EXPM, TALLY, RUNS>(a, len, region, z, track_off, items, item, lane, tb);
+    }
+}
+
+// ---- packing of region data (once per smoc_set_regions) ------------------------
+template <int STORE>
+__global__ void __launch_bounds__(256) k_pack(smoc_pack_args a)
+{
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)a.R * a.F * a.Gp;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int G = a.G, Gp = a.Gp, F = a.F;
+    const int g = (int)(i % Gp);
+    const int64_t rf = i / Gp;
+    using raw_t = typename Tup<STORE>::raw;
+    if (g >= G) {  // padding lanes compute on sigT = 1, zero source
+        reinterpret_cast<raw_t *>(a.tuples)[i] = Tup<STORE>::one();
+        return;
+    }
+    const int f = (int)(rf % F);
+    const int64_t r = rf / F;
+    const float sig = a.sigT[r * G + g];
+    const float *src = a.fine_source + r * F * G;
+    const float ylo = f > 0 ? src[(int64_t)(f - 1) * G + g] : 0.f;
+    const float ymid = src[(int64_t)f * G + g];
+    const float yhi = f < F - 1 ? src[(int64_t)(f + 1) * G + g] : 0.f;
+    if constexpr (STORE == SMOC_STORE_F32) {
+        // literal source fit coefficients, IEEE divisions (kernel.c:128-189)
+        float c1, c2;
+        if (f == 0) {
+            c1 = (yhi - ymid) / kDz;
+            c2 = 0.f;
+        } else if (f == F - 1) {
+            c1 = (ymid - ylo) / kDz;
+            c2 = 0.f;
+        } else {
+            c1 = (ylo - yhi) / (2.f * kDz);
+            c2 = (ylo - 2.f * ymid + yhi) / (2.f * kDz * kDz);
+        }
+        reinterpret_cast<float4 *>(a.tuples)[i] = make_float4(sig, ymid, c1, c2);
+    } else if constexpr (STORE == SMOC_STORE_BF16) {
+        reinterpret_cast<uint2 *>(a.tuples)[i] =
+            make_uint2(f_to_bf(sig) | (f_to_bf(ylo) << 16), f_to_bf(ymid) | (f_to_bf(yhi) << 16));
+    } else {
+        reinterpret_cast<uint2 *>(a.tuples)[i] =
+            make_uint2(f_to_h(sig) | (f_to_h(ylo) << 16), f_to_h(ymid) | (f_to_h(yhi) << 16));
+    }
+}
+
+// ---- resets --------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_reset(T *flux, const float *init, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flux[i] = init ? (T)init[i] : (T)0;
+}
+
+__global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+// ---- deterministic segmented reduction over run partials -------------------------
+// One wave per ((region, interval), group slice); the runs of a (region,
+// interval) are summed in ascending run-id order, whatever the launch shape.
+template <int TALLY, int GPL>
+__global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int64_t item = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * wpb + wave;
+    if (item >= a.n_fsr * a.slices)
+        return;
+    const int64_t i = item / a.slices;
+    const int sl = (int)(item - i * a.slices);
+    const int Gp = a.Gp;
+    const int64_t rb = a.fsr_run_off[i], re = a.fsr_run_off[i + 1];
+#pragma unroll
+    for (int j = 0; j < GPL; j++) {
+        const int g = sl * 64 * GPL + j * 64 + lane;
+        const int64_t o = i * Gp + g;
+        if constexpr (TALLY == SMOC_TALLY_DET_F64) {
+            const double *part = static_cast<const double *>(a.part);
+            double s = a.init ? (double)a.init[o] : 0.0;
+            int64_t r = rb;
+            for (; r + 4 <= re; r += 4) {
+                const double v0 = part[a.fsr_run_idx[r] * Gp + g];
+                const double v1 = part[a.fsr_run_idx[r + 1] * Gp + g];
+                const double v2 = part[a.fsr_run_idx[r + 2] * Gp + g];
+                const double v3 = part[a.fsr_run_idx[r + 3] * Gp + g];
+                s += v0;
+                s += v1;
+                s += v2;
+                s += v3;
+            }
+            for (; r < re; ++r)
+                s += part[a.fsr_run_idx[r] * Gp + g];
+            static_cast<double *>(a.flux)[o] = s;
+        } else {
+#pragma clang fp contract(off)
+            const float2 *part = static_cast<const float2 *>(a.part);
+            float s = a.init ? a.init[o] : 0.f, c = 0.f;
+            for (int64_t r = rb; r < re; ++r) {
+                const float2 p = part[a.fsr_run_idx[r] * Gp + g];
+                float y = p.x - c;
+                float t = s + y;
+                c = (t - s) - y;
+                s = t;
+                y = -p.y - c;
+                t = s + y;
+                c = (t - s) - y;
+                s = t;
+            }
+            static_cast<float *>(a.flux)[o] = s - c;
+        }
+    }
+}
+
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+int launch_sweep_r(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
+{
+    if (a->queue)
+        hipLaunchKernelGGL((k_sweep_persistent<STORE, GPL, EXPM, TALLY, RUNS>), dim3(grid),
+                           dim3(64 * wpb), 0, s, *a, a->len, a->region, a->z, a->track_off,
+                           a->items);
+    else
+        hipLaunchKernelGGL((k_sweep<STORE, GPL, EXPM, TALLY, RUNS>), dim3(grid), dim3(64 * wpb),
+                           0, s, *a, a->len, a->region, a->z, a->track_off, a->items);
+    return (int)hipGetLastError();
+}
+
+template <int STORE, int GPL, int EXPM, int TALLY>
+int launch_sweep_t(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
+{
+    if (a->runs)
+        return launch_sweep_r<STORE, GPL, EXPM, TALLY, true>(a, wpb, grid, s);
+    return launch_sweep_r<STORE, GPL, EXPM, TALLY, false>(a, wpb, grid, s);
+}
+
+template <int STORE, int GPL, int EXPM>
+int launch_sweep_tally(const smoc_sweep_args *a, int tally, int wpb, int grid, hipStream_t s)
+{
+    switch (tally) {
+    case SMOC_TALLY_ATOMIC_F32:
+        return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F32>(a, wpb, grid, s);
+    case SMOC_TALLY_ATOMIC_F64:
+        return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F64>(a, wpb, grid, s);
+    case SMOC_TALLY_DET_F64:
+        return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_DET_F64>(a, wpb, grid, s);
+    case SMOC_TALLY_DET_KAHAN:
+        return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_DET_KAHAN>(a, wpb, grid, s);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+template <int STORE, int GPL>
+int launch_sweep_exp(const smoc_sweep_args *a, int expm, int tally, int wpb, int grid,
+                     hipStream_t s)
+{
+    switch (expm) {
+    case SMOC_EXP_TABLE:
+        return launch_sweep_tally<STORE, GPL, SMOC_EXP_TABLE>(a, tally, wpb, grid, s);
+    case SMOC_EXP_ACCURATE:
+        return launch_sweep_tally<STORE, GPL, SMOC_EXP_ACCURATE>(a, tally, wpb, grid, s);
+    case SMOC_EXP_NATIVE:
+        return launch_sweep_tally<STORE, GPL, SMOC_EXP_NATIVE>(a, tally, wpb, grid, s);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+template <int STORE>
+int launch_sweep_gpl(const smoc_sweep_args *a, int gpl, int expm, int tally, int wpb, int grid,
+                     hipStream_t s)
+{
+    if (gpl == 2)
+        return launch_sweep_exp<STORE, 2>(a, expm, tally, wpb, grid, s);
+    return launch_sweep_exp<STORE, 1>(a, expm, tally, wpb, grid, s);
+}
+
+inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+extern "C" int smoc_tuple_bytes(int storage)
+{
+    return storage == SMOC_STORE_F32 ? 16 : 8;
+}
+
+extern "C" int smoc_launch_pack(const smoc_pack_args *a, int storage, void *stream)
+{
+    const int64_t n = (int64_t)a->R * a->F * a->Gp;
+    if (n == 0)
+        return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = blocks_for(n, 256);
+    switch (storage) {
+    case SMOC_STORE_F32: hipLaunchKernelGGL(k_pack<SMOC_STORE_F32>, dim3(nb), dim3(256), 0, s, *a); break;
+    case SMOC_STORE_BF16: hipLaunchKernelGGL(k_pack<SMOC_STORE_BF16>, dim3(nb), dim3(256), 0, s, *a); break;
+    case SMOC_STORE_F16: hipLaunchKernelGGL(k_pack<SMOC_STORE_F16>, dim3(nb), dim3(256), 0, s, *a); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_reset(void *flux, const float *init, int64_t n, int f64, void *stream)
+{
+    if (n == 0)
+        return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = blocks_for(n, 256);
+    if (f64)
+        hipLaunchKernelGGL(k_reset<double>, dim3(nb), dim3(256), 0, s, static_cast<double *>(flux), init, n);
+    else
+        hipLaunchKernelGGL(k_reset<float>, dim3(nb), dim3(256), 0, s, static_cast<float *>(flux), init, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream)
+{
+    if (n == 0)
+        return 0;
+    hipLaunchKernelGGL(k_zero_i64, dim3(blocks_for(n, 64)), dim3(64), 0,
+                       static_cast<hipStream_t>(stream), p, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl, int exp_mode,
+                                 int tally, int wpb, int grid_blocks, void *stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (grid_blocks <= 0)
+        return 0;
+    switch (storage) {
+    case SMOC_STORE_F32: return launch_sweep_gpl<SMOC_STORE_F32>(a, gpl, exp_mode, tally, wpb, grid_blocks, s);
+    case SMOC_STORE_BF16: return launch_sweep_gpl<SMOC_STORE_BF16>(a, gpl, exp_mode, tally, wpb, grid_blocks, s);
+    case SMOC_STORE_F16: return launch_sweep_gpl<SMOC_STORE_F16>(a, gpl, exp_mode, tally, wpb, grid_blocks, s);
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int wpb,
+                                  void *stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t n = a->n_fsr * a->slices;
+    if (n == 0)
+        return 0;
+    const unsigned nb = blocks_for(n, wpb);
+    const dim3 blk(64 * wpb);
+    if (tally == SMOC_TALLY_DET_F64) {
+        if (gpl == 2)
+            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_F64, 2>), dim3(nb), blk, 0, s, *a);
+        else
+            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_F64, 1>), dim3(nb), blk, 0, s, *a);
+    } else if (tally == SMOC_TALLY_DET_KAHAN) {
+        if (gpl == 2)
+            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_KAHAN, 2>), dim3(nb), blk, 0, s, *a);
+        else
+            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_KAHAN, 1>), dim3(nb), blk, 0, s, *a);
+    } else {
+        return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
