@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--wpb", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-segments", type=int, default=4_000_000)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (counter "
+                         "calibration) and print its bandwidth")
     return ap.parse_args()
 
 
@@ -95,6 +98,12 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                           "(the sweep shards over tracks, see DESIGN.md)"}))
         sys.exit(2)
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        ms = smoc.probe_copy(nbytes, reps=5)
+        print(json.dumps({"copy_probe_bytes": nbytes, "mean_ms": ms,
+                          "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
     G, S, L = args.groups, args.segments, args.track_len
     t0 = time.time()
     prob = smoc.generate(S, G=G, R=6750, F=5, seed=42, track_len=L)

@@ -92,9 +92,11 @@ typedef struct {
 /* Launch configuration.  Zero fields mean "choose": the library's default
  * or, with autotune = 1, the fastest candidate measured on this problem.
  * None of these fields changes results except chunk (splitting a track into
- * chunks re-associates its psi recurrence; see smoc_run). */
+ * chunks re-associates its psi recurrence through the affine scan, a
+ * rounding-level change). */
 typedef struct {
-    int seg_per_block;   /* -p: segments a workgroup sweeps per work item     */
+    int seg_per_block;   /* -p: segments per work item; used as chunk when
+                            chunk is 0 (0 = whole tracks)                     */
     int groups_per_lane; /* 1 or 2 energy groups per lane                     */
     int waves_per_block; /* wavefronts per workgroup (1..16)                   */
     int chunk;           /* split tracks longer than this many segments into
@@ -106,8 +108,10 @@ typedef struct {
 
 typedef struct {
     double total_ms;   /* whole run, HIP events on the context stream        */
-    double reset_ms;   /* fine_flux reset (and queue counters)              */
-    double sweep_ms;   /* attenuation sweep kernel(s)                        */
+    double reset_ms;   /* fine_flux reset                                    */
+    double xfer_ms;    /* chunked runs: transfer pass + psi scan              */
+    double sweep_ms;   /* attenuation sweep kernel (the whole graph with
+                          use_graph, whose phases are not timed apart)        */
     double reduce_ms;  /* deterministic segmented reduction (DET_* only)     */
     int64_t segments;  /* segments swept                                     */
     int64_t work_items;/* (track or chunk) x group-slice items launched      */
@@ -168,6 +172,10 @@ int smoc_tracks_lognormal(uint64_t seed, int64_t T, double median, double sigma,
  * key, in place.  Tracks stay defined as contiguous ranges of the stream. */
 int smoc_sort_segments(int64_t S, int R, int F, float *len, int32_t *region, float *z);
 
+/* Host threads for the generator and sort (reference -t, src/cpu/io.c:114-120);
+ * n <= 0 leaves the OpenMP default. */
+int smoc_set_host_threads(int n);
+
 /* Fixed-index-order FP64 sums, the result checksums. */
 double smoc_checksum_f64(const double *x, int64_t n);
 double smoc_checksum_f32(const float *x, int64_t n);
@@ -176,6 +184,12 @@ double smoc_checksum_f32(const float *x, int64_t n);
 int smoc_device_count(int *n);
 /* Device name and compute-unit count for reports. */
 int smoc_device_info(int device, char *name, int name_cap, int *compute_units);
+
+/* Measurement utility (no reference counterpart): copies `bytes` bytes
+ * between two device buffers with a plain vectorised copy kernel `reps`
+ * times and returns the mean kernel time.  Used to calibrate HBM traffic
+ * counters and to state the achievable copy bandwidth next to the roofline. */
+int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms);
 
 int smoc_create(smoc_ctx **ctx, const smoc_params *p);
 int smoc_destroy(smoc_ctx *ctx);

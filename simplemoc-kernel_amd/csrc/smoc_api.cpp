@@ -71,13 +71,34 @@ template <typename T> struct DevBuf {
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Chunking of long tracks for one chunk length C.
+struct Plan {
+    int chunk = 0;
+    int64_t n_items = 0, n_xfer = 0;
+    DevBuf<int64_t> items;           // [n_items][4]
+    DevBuf<int64_t> xfer_list;       // items needing a transfer pass
+    DevBuf<int64_t> track_item_off;  // [T+1]
+    DevBuf<float> chunk_psi;         // [n_items][Gp]
+    DevBuf<float> xfer;              // [n_items][2][Gp]
+    void release()
+    {
+        items.release();
+        xfer_list.release();
+        track_item_off.release();
+        chunk_psi.release();
+        xfer.release();
+        chunk = 0;
+        n_items = n_xfer = 0;
+    }
+};
+
 }  // namespace
 
 struct smoc_ctx {
     smoc_params p{};
     int Gp = 0;  // padded group stride
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int cu_count = 0;
 
     // region data
@@ -103,7 +124,14 @@ struct smoc_ctx {
     DevBuf<int64_t> track_run_off, fsr_run_off, fsr_run_idx;
     DevBuf<unsigned char> part;
 
-    DevBuf<int64_t> queue;
+    DevBuf<int64_t> queue;  // persistent work counters
+    std::vector<int64_t> h_track_off;
+    int64_t max_track = 0;
+    Plan plan;
+    hipGraphExec_t graph_exec = nullptr;
+    smoc_launch graph_launch{};
+    smoc_launch tuned{};
+    bool tuned_valid = false;
     bool ran = false;
 
     int64_t n_fsr() const { return (int64_t)p.regions * p.fine_axial; }
@@ -148,6 +176,51 @@ int smoc_device_info(int device, char *name, int name_cap, int *compute_units)
     return SMOC_OK;
 }
 
+int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
+{
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1 || !mean_ms) {
+        smoc_set_error("smoc_probe_copy: bytes must be a positive multiple of 16, reps >= 1");
+        return SMOC_ERR_ARG;
+    }
+    int ndev = 0;
+    int rc = smoc_device_count(&ndev);
+    if (rc != SMOC_OK)
+        return rc;
+    if (device < 0 || device >= ndev) {
+        smoc_set_error("smoc_probe_copy: device %d not available", device);
+        return SMOC_ERR_NODEV;
+    }
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    DevBuf<unsigned char> a, b;
+    rc = a.alloc((size_t)bytes);
+    if (rc == SMOC_OK)
+        rc = b.alloc((size_t)bytes);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemset(a.p, 1, (size_t)bytes));
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int grid = prop.multiProcessorCount * 8;
+    LAUNCHCHK(smoc_launch_copy(a.p, b.p, bytes / 16, grid, s));  // warm-up
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++)
+        LAUNCHCHK(smoc_launch_copy(a.p, b.p, bytes / 16, grid, s));
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *mean_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return SMOC_OK;
+}
+
 int smoc_create(smoc_ctx **out, const smoc_params *p)
 {
     if (!out || !p) {
@@ -186,7 +259,7 @@ int smoc_create(smoc_ctx **out, const smoc_params *p)
     c->p = *p;
     c->Gp = round_up(p->egroups, 128);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int i = 0; e == hipSuccess && i < 4; i++)
+    for (int i = 0; e == hipSuccess && i < 5; i++)
         e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess) {
         hipDeviceProp_t prop;
@@ -242,6 +315,9 @@ int smoc_destroy(smoc_ctx *c)
     c->fsr_run_idx.release();
     c->part.release();
     c->queue.release();
+    c->plan.release();
+    if (c->graph_exec)
+        (void)hipGraphExecDestroy(c->graph_exec);
     for (auto &ev : c->ev)
         if (ev)
             (void)hipEventDestroy(ev);
@@ -315,6 +391,7 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
     }
     c->regions_set = true;
     c->ran = false;
+    c->tuned_valid = false;
     return SMOC_OK;
 }
 
@@ -366,7 +443,7 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     if (rc == SMOC_OK)
         rc = c->fsr_run_idx.alloc((size_t)std::max<int64_t>(n_runs, 1));
     if (rc == SMOC_OK)
-        rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * 8);
+        rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
     if (rc != SMOC_OK)
         return rc;
     HIPCHK(hipMemcpy(c->track_run_off.p, tro.data(), sizeof(int64_t) * (T + 1),
@@ -455,6 +532,16 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
     c->S = S;
     c->T = T;
     c->n_runs = 0;
+    c->h_track_off.assign(track_off, track_off + T + 1);
+    c->max_track = 0;
+    for (int64_t t = 0; t < T; t++)
+        c->max_track = std::max<int64_t>(c->max_track, track_off[t + 1] - track_off[t]);
+    c->plan.release();
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->tuned_valid = false;
     if (c->det()) {
         rc = build_runs(c, S, region, z, T, track_off);
         if (rc != SMOC_OK)
@@ -474,9 +561,334 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
         d.groups_per_lane = c->p.egroups > 64 ? 2 : 1;
     if (d.waves_per_block < 1 || d.waves_per_block > 16)
         d.waves_per_block = 4;
-    if (d.seg_per_block <= 0)
-        d.seg_per_block = 100;
+    if (d.chunk < 0)
+        d.chunk = 0;
+    if (d.chunk == 0 && d.seg_per_block > 0)
+        d.chunk = d.seg_per_block;  // -p: segments per work item
+    if (d.chunk > 0 && d.chunk >= c->max_track)
+        d.chunk = 0;  // nothing to split
+    d.persistent = d.persistent ? 1 : 0;
+    d.use_graph = d.use_graph ? 1 : 0;
     *l = d;
+}
+
+// Work items of a chunked sweep: every track becomes ceil(len / C) chunks
+// (an empty track one empty chunk); chunks that are not their track's last
+// need a transfer pass.
+static int build_plan(smoc_ctx *c, int C)
+{
+    Plan &pl = c->plan;
+    std::vector<int64_t> items, xl, tio((size_t)c->T + 1);
+    const std::vector<int64_t> &off = c->h_track_off;
+    int64_t n = 0;
+    for (int64_t t = 0; t < c->T; t++) {
+        tio[t] = n;
+        const int64_t b = off[t], e = off[t + 1];
+        const int64_t nc = std::max<int64_t>(1, (e - b + C - 1) / C);
+        for (int64_t k = 0; k < nc; k++, n++) {
+            const int64_t cb = b + k * C, ce = std::min<int64_t>(e, cb + C);
+            items.push_back(t);
+            items.push_back(cb);
+            items.push_back(ce);
+            items.push_back(cb);  // first run id: runs are segments (checked by caller)
+            if (k + 1 < nc)
+                xl.push_back(n);
+        }
+    }
+    tio[c->T] = n;
+    int rc = pl.items.alloc(items.size());
+    if (rc == SMOC_OK)
+        rc = pl.xfer_list.alloc(std::max<size_t>(xl.size(), 1));
+    if (rc == SMOC_OK)
+        rc = pl.track_item_off.alloc(tio.size());
+    if (rc == SMOC_OK)
+        rc = pl.chunk_psi.alloc((size_t)n * c->Gp);
+    if (rc == SMOC_OK)
+        rc = pl.xfer.alloc((size_t)std::max<int64_t>(n, 1) * 2 * c->Gp);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpy(pl.items.p, items.data(), sizeof(int64_t) * items.size(),
+                     hipMemcpyHostToDevice));
+    if (!xl.empty())
+        HIPCHK(hipMemcpy(pl.xfer_list.p, xl.data(), sizeof(int64_t) * xl.size(),
+                         hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl.track_item_off.p, tio.data(), sizeof(int64_t) * tio.size(),
+                     hipMemcpyHostToDevice));
+    pl.chunk = C;
+    pl.n_items = n;
+    pl.n_xfer = (int64_t)xl.size();
+    return SMOC_OK;
+}
+
+// Enqueue one whole sweep on the context stream.  ev (4 events) may be NULL
+// (graph capture): reset | transfer + scan | sweep | reduce.
+static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
+{
+    const int G = c->p.egroups;
+    const int gpl = l.groups_per_lane, wpb = l.waves_per_block;
+    const int slices = (G + 64 * gpl - 1) / (64 * gpl);
+    const int64_t nrow = c->n_fsr();
+    const bool chunked = l.chunk > 0;
+    if (chunked && c->det() && c->runs) {
+        smoc_set_error("smoc_run: chunked sweeps of a region-sorted stream need an atomic "
+                       "tally (deterministic runs would cross chunk boundaries)");
+        return SMOC_ERR_ARG;
+    }
+    if (chunked && c->plan.chunk != l.chunk) {
+        int rc = build_plan(c, l.chunk);
+        if (rc != SMOC_OK)
+            return rc;
+    }
+    if (l.persistent && !c->queue.p) {
+        int rc = c->queue.alloc(2);
+        if (rc != SMOC_OK)
+            return rc;
+    }
+
+    smoc_sweep_args a{};
+    a.tuples = c->tuples.p;
+    a.len = c->len.p;
+    a.region = c->region.p;
+    a.z = c->z.p;
+    a.track_off = c->track_off.p;
+    a.psi_in = c->psi_in.p;
+    a.psi_out = c->psi_out.p;
+    a.flux = c->det() ? (void *)c->part.p : (void *)c->flux.p;
+    a.track_run_off = c->track_run_off.p;
+    a.table = c->table.p;
+    a.tab_dx = c->tab_dx;
+    a.tab_max = c->tab_max;
+    a.tab_n = c->tab_n;
+    a.G = G;
+    a.Gp = c->Gp;
+    a.F = c->p.fine_axial;
+    a.slices = slices;
+    a.runs = c->runs ? 1 : 0;
+    // persistent grids: as many workgroups as can be resident (8 waves per
+    // SIMD, 4 SIMDs per CU)
+    const int64_t resident_blocks = (int64_t)c->cu_count * std::max(1, 32 / wpb);
+    auto grid_for = [&](int64_t n_entries) {
+        int64_t g = (n_entries * slices + wpb - 1) / wpb;
+        if (l.persistent)
+            g = std::min<int64_t>(g, resident_blocks);
+        return g;
+    };
+
+    if (ev)
+        HIPCHK(hipEventRecord(ev[0], c->stream));
+    if (!c->det())
+        LAUNCHCHK(smoc_launch_reset(c->flux.p, c->flux_init.p, nrow * c->Gp,
+                                    c->flux_f64() ? 1 : 0, c->stream));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[1], c->stream));
+    if (chunked) {
+        Plan &pl = c->plan;
+        a.items = pl.items.p;
+        a.chunk_psi = pl.chunk_psi.p;
+        a.xfer = pl.xfer.p;
+        if (pl.n_xfer > 0) {
+            smoc_sweep_args x = a;
+            x.xfer_mode = 1;
+            x.item_list = pl.xfer_list.p;
+            x.chunk_psi = nullptr;
+            x.n_items = pl.n_xfer;
+            if (l.persistent) {
+                x.queue = c->queue.p;
+                LAUNCHCHK(smoc_launch_zero_i64(c->queue.p, 1, c->stream));
+            }
+            LAUNCHCHK(smoc_launch_sweep(&x, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
+                                        (int)grid_for(pl.n_xfer), c->stream));
+        }
+        smoc_scan_args sa{pl.track_item_off.p, pl.xfer.p, c->psi_in.p, pl.chunk_psi.p, c->T,
+                          c->Gp, slices};
+        LAUNCHCHK(smoc_launch_scan(&sa, gpl, c->stream));
+        a.n_items = pl.n_items;
+    } else {
+        a.n_items = c->T;
+    }
+    if (ev)
+        HIPCHK(hipEventRecord(ev[2], c->stream));
+    if (l.persistent) {
+        a.queue = c->queue.p + 1;
+        LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
+    }
+    LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
+                                (int)grid_for(a.n_items), c->stream));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[3], c->stream));
+    if (c->det()) {
+        smoc_reduce_args r{};
+        r.part = c->part.p;
+        r.fsr_run_off = c->fsr_run_off.p;
+        r.fsr_run_idx = c->fsr_run_idx.p;
+        r.init = c->flux_init.p;
+        r.flux = c->flux.p;
+        r.n_fsr = nrow;
+        r.Gp = c->Gp;
+        r.slices = slices;
+        r.single = c->runs ? 0 : 1;
+        LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, 4, c->stream));
+    }
+    if (ev)
+        HIPCHK(hipEventRecord(ev[4], c->stream));
+    return SMOC_OK;
+}
+
+static bool same_launch(const smoc_launch &x, const smoc_launch &y)
+{
+    return x.groups_per_lane == y.groups_per_lane && x.waves_per_block == y.waves_per_block &&
+           x.chunk == y.chunk && x.persistent == y.persistent;
+}
+
+// One run, timed with events on the context stream; with use_graph the
+// sequence is captured once into a hipGraph and replayed.
+static int run_once(smoc_ctx *c, const smoc_launch &l, smoc_timing *tm)
+{
+    if (l.use_graph) {
+        if (!c->graph_exec || !same_launch(c->graph_launch, l)) {
+            if (c->graph_exec) {
+                (void)hipGraphExecDestroy(c->graph_exec);
+                c->graph_exec = nullptr;
+            }
+            // plan buffers must exist before capture (no allocation inside)
+            if (l.chunk > 0 && c->plan.chunk != l.chunk) {
+                int rc = build_plan(c, l.chunk);
+                if (rc != SMOC_OK)
+                    return rc;
+            }
+            if (l.persistent && !c->queue.p) {
+                int rc = c->queue.alloc(2);
+                if (rc != SMOC_OK)
+                    return rc;
+            }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_run(c, l, nullptr);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != SMOC_OK) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                smoc_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+                return SMOC_ERR_HIP;
+            }
+            e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->graph_exec = nullptr;
+                smoc_set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+                return SMOC_ERR_HIP;
+            }
+            c->graph_launch = l;
+        }
+        HIPCHK(hipEventRecord(c->ev[0], c->stream));
+        HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+        HIPCHK(hipEventRecord(c->ev[4], c->stream));
+        HIPCHK(hipEventSynchronize(c->ev[4]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[4]));
+        *tm = smoc_timing{};
+        tm->total_ms = ms;
+        tm->sweep_ms = ms;  // one graph: phases are not timed separately
+        return SMOC_OK;
+    }
+    int rc = enqueue_run(c, l, c->ev);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipEventSynchronize(c->ev[4]));
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; i++)
+        HIPCHK(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
+    *tm = smoc_timing{};
+    tm->reset_ms = ms[0];
+    tm->xfer_ms = ms[1];
+    tm->sweep_ms = ms[2];
+    tm->reduce_ms = ms[3];
+    tm->total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
+    return SMOC_OK;
+}
+
+// Measures candidate launch shapes on the loaded problem (each run is a
+// complete, correct sweep) and keeps the fastest: first the track split and
+// persistence, then groups per lane, then waves per workgroup.
+static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
+{
+    auto measure = [&](const smoc_launch &l, double *ms) {
+        smoc_timing t{};
+        int rc = run_once(c, l, &t);  // warm-up (plans, graphs, caches)
+        if (rc != SMOC_OK)
+            return rc;
+        double best = 1e30;
+        for (int r = 0; r < 2; r++) {
+            rc = run_once(c, l, &t);
+            if (rc != SMOC_OK)
+                return rc;
+            best = std::min(best, t.total_ms);
+        }
+        *ms = best;
+        return SMOC_OK;
+    };
+    smoc_launch best = base;
+    best.autotune = 0;
+    best.chunk = 0;
+    best.persistent = 0;
+    double best_ms = 0.0;
+    int rc = measure(best, &best_ms);
+    if (rc != SMOC_OK)
+        return rc;
+    auto consider = [&](smoc_launch cand) {
+        double ms = 0.0;
+        fill_launch(c, &cand, &cand);
+        if (same_launch(cand, best))
+            return SMOC_OK;
+        int r = measure(cand, &ms);
+        if (r == SMOC_ERR_ARG)  // combination not supported for this problem
+            return SMOC_OK;
+        if (r != SMOC_OK)
+            return r;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cand;
+        }
+        return SMOC_OK;
+    };
+    const int64_t mean_len = c->T > 0 ? c->S / c->T : 0;
+    {
+        smoc_launch cand = best;
+        cand.persistent = 1;
+        if ((rc = consider(cand)) != SMOC_OK)
+            return rc;
+    }
+    if (c->max_track > 4 * std::max<int64_t>(mean_len, 1)) {
+        const smoc_launch base_split = best;
+        for (int C : {1024, 2048, 4096, 8192, 16384}) {
+            for (int pers = 0; pers < 2; pers++) {
+                smoc_launch cand = base_split;
+                cand.chunk = C;
+                cand.seg_per_block = 0;
+                cand.persistent = pers;
+                if ((rc = consider(cand)) != SMOC_OK)
+                    return rc;
+            }
+        }
+    }
+    if (c->p.egroups > 64) {
+        smoc_launch cand = best;
+        cand.groups_per_lane = 3 - best.groups_per_lane;
+        if ((rc = consider(cand)) != SMOC_OK)
+            return rc;
+    }
+    for (int w : {1, 2, 8}) {
+        smoc_launch cand = best;
+        cand.waves_per_block = w;
+        if ((rc = consider(cand)) != SMOC_OK)
+            return rc;
+    }
+    best.use_graph = base.use_graph;
+    *best_out = best;
+    return SMOC_OK;
 }
 
 int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
@@ -492,87 +904,29 @@ int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
     HIPCHK(hipSetDevice(c->p.device));
     smoc_launch l;
     fill_launch(c, launch, &l);
-    const int G = c->p.egroups;
-    const int gpl = l.groups_per_lane;
-    const int slices = (G + 64 * gpl - 1) / (64 * gpl);
-    const int64_t nrow = c->n_fsr();
-
-    smoc_sweep_args a{};
-    a.tuples = c->tuples.p;
-    a.len = c->len.p;
-    a.region = c->region.p;
-    a.z = c->z.p;
-    a.track_off = c->track_off.p;
-    a.psi_in = c->psi_in.p;
-    a.psi_out = c->psi_out.p;
-    a.flux = c->det() ? (void *)c->part.p : (void *)c->flux.p;
-    a.track_run_off = c->track_run_off.p;
-    a.table = c->table.p;
-    a.items = nullptr;
-    a.chunk_psi = nullptr;
-    a.n_items = c->T;
-    a.queue = nullptr;
-    a.tab_dx = c->tab_dx;
-    a.tab_max = c->tab_max;
-    a.tab_n = c->tab_n;
-    a.G = G;
-    a.Gp = c->Gp;
-    a.F = c->p.fine_axial;
-    a.slices = slices;
-    a.runs = c->runs ? 1 : 0;
-
-    const int64_t n_work = c->T * slices;
-    int64_t grid64 = (n_work + l.waves_per_block - 1) / l.waves_per_block;
-    if (l.persistent) {
-        if (!c->queue.p) {
-            int rc = c->queue.alloc(1);
+    if (l.autotune) {
+        if (!c->tuned_valid) {
+            int rc = autotune(c, l, &c->tuned);
             if (rc != SMOC_OK)
                 return rc;
+            c->tuned_valid = true;
         }
-        a.queue = c->queue.p;
-        grid64 = std::min<int64_t>(grid64, (int64_t)c->cu_count * (32 / l.waves_per_block > 0 ? 32 / l.waves_per_block : 1));
+        const int graph = l.use_graph;
+        l = c->tuned;
+        l.use_graph = graph;
+        l.autotune = 1;
     }
-    if (grid64 > INT32_MAX) {
-        smoc_set_error("smoc_run: %lld workgroups exceed the grid limit", (long long)grid64);
-        return SMOC_ERR_ARG;
-    }
-
-    HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    if (!c->det())
-        LAUNCHCHK(smoc_launch_reset(c->flux.p, c->flux_init.p, nrow * c->Gp, c->flux_f64() ? 1 : 0,
-                                    c->stream));
-    if (a.queue)
-        LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
-    HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally,
-                                l.waves_per_block, (int)grid64, c->stream));
-    HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    if (c->det()) {
-        smoc_reduce_args r{};
-        r.part = c->part.p;
-        r.fsr_run_off = c->fsr_run_off.p;
-        r.fsr_run_idx = c->fsr_run_idx.p;
-        r.init = c->flux_init.p;
-        r.flux = c->flux.p;
-        r.n_fsr = nrow;
-        r.Gp = c->Gp;
-        r.slices = slices;
-        LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, 4, c->stream));
-    }
-    HIPCHK(hipEventRecord(c->ev[3], c->stream));
-    HIPCHK(hipEventSynchronize(c->ev[3]));
+    smoc_timing tm{};
+    int rc = run_once(c, l, &tm);
+    if (rc != SMOC_OK)
+        return rc;
     c->ran = true;
     if (timing) {
-        float ms[3] = {0.f, 0.f, 0.f};
-        HIPCHK(hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]));
-        HIPCHK(hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]));
-        HIPCHK(hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]));
-        timing->reset_ms = ms[0];
-        timing->sweep_ms = ms[1];
-        timing->reduce_ms = ms[2];
-        timing->total_ms = (double)ms[0] + ms[1] + ms[2];
+        const int G = c->p.egroups;
+        const int slices = (G + 64 * l.groups_per_lane - 1) / (64 * l.groups_per_lane);
+        *timing = tm;
         timing->segments = c->S;
-        timing->work_items = n_work;
+        timing->work_items = (l.chunk > 0 ? c->plan.n_items : c->T) * slices;
         timing->used = l;
     }
     return SMOC_OK;

@@ -11,6 +11,7 @@
 #include "smoc_internal.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -290,6 +291,13 @@ int smoc_sort_segments(int64_t S, int R, int F, float *len, int32_t *region, flo
     memcpy(region, tr, (size_t)S * sizeof(int32_t));
     memcpy(z, tz, (size_t)S * sizeof(float));
     free(cnt), free(key), free(tl), free(tz), free(tr);
+    return SMOC_OK;
+}
+
+int smoc_set_host_threads(int n)
+{
+    if (n > 0)
+        omp_set_num_threads(n);
     return SMOC_OK;
 }
 

@@ -37,9 +37,13 @@ typedef struct {
     const int64_t *track_run_off;/* DET_*: first run id of each track [T+1]  */
     const float *table;          /* exp table, 2N floats (EXP_TABLE)          */
     /* chunked sweeps (tracks split into chunks) */
-    const int64_t *items;        /* NULL: item i = track i; else [n][3]: track, begin, end */
-    const float *chunk_psi;      /* [n_items][Gp] incoming psi of each chunk, or NULL */
-    int64_t n_items;             /* tracks or chunks */
+    const int64_t *items;        /* NULL: item i = track i; else [n][4]: track, begin,
+                                    end, first run id */
+    const int64_t *item_list;    /* NULL: work entry w = item w; else item ids */
+    const float *chunk_psi;      /* [items][Gp] incoming psi of each chunk, or NULL */
+    float *xfer;                 /* transfer pass output [items][2][Gp]: B, M */
+    int64_t n_items;             /* work entries: tracks, chunks or list length */
+    int xfer_mode;               /* 1: transfer pass (no tally) */
     int64_t *queue;              /* persistent mode: work counter, or NULL    */
     float tab_dx, tab_max;
     int tab_n;
@@ -47,6 +51,15 @@ typedef struct {
     int runs;                    /* 1: sum runs of equal (region, interval) in
                                     registers; 0: every segment is a run */
 } smoc_sweep_args;
+
+typedef struct {
+    const int64_t *track_item_off; /* [T+1] first chunk of each track */
+    const float *xfer;             /* [items][2][Gp] */
+    const float *psi_in;           /* [T][Gp] */
+    float *chunk_psi;              /* [items][Gp] */
+    int64_t T;
+    int Gp, slices;
+} smoc_scan_args;
 
 typedef struct {
     const float *sigT;        /* [R][G] FP32 master */
@@ -63,6 +76,7 @@ typedef struct {
     void *flux;                  /* out fine_flux [R*F][Gp]: f64 (DET_F64) / f32 (DET_KAHAN) */
     int64_t n_fsr;
     int Gp, slices;
+    int single;                  /* partials are single FP32 tallies */
 } smoc_reduce_args;
 
 /* kernel launchers (smoc_kernels.hip); return 0 or a hipError_t value */
@@ -72,7 +86,9 @@ int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl, int exp_mo
                       int waves_per_block, int grid_blocks, void *stream);
 int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_per_block,
                        void *stream);
+int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream);
 int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
+int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);
 /* bytes per packed tuple for a storage type */
 int smoc_tuple_bytes(int storage);
 

@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "smoc.h"
 #include "smoc_internal.h"
 
@@ -226,7 +228,7 @@ __device__ __forceinline__ EvTerms ev_terms(float tau, const ExpTab &tb)
 // (weight * flux_integral) and updates psi.
 template <int EXPM>
 __device__ __forceinline__ float attenuate(float sig, float q0, float q1, float q2, float ds,
-                                           float &psi, const ExpTab &tb)
+                                           float &psi, float &one_m_ev, const ExpTab &tb)
 {
     const float tau = sig * ds;
     const EvTerms x = ev_terms<EXPM>(tau, tb);
@@ -238,12 +240,15 @@ __device__ __forceinline__ float attenuate(float sig, float q0, float q1, float 
     const float flux_integral = (q0 * tau + (sig * psi - q0) * ev) * i2 + q1 * SMOC_MU * reuse +
                                 q2 * SMOC_MU2 * x.poly * (i2 * i2 * (1.f / 3.f));
     psi = q0 * ev * i1 + q1 * SMOC_MU * x.tme * i2 + q2 * SMOC_MU2 * reuse + psi * x.one_m_ev;
+    one_m_ev = x.one_m_ev;
     return SMOC_WEIGHT * flux_integral;
 }
 
 // ---- tally accumulators ------------------------------------------------------
-template <int TALLY> struct Acc;
-template <> struct Acc<SMOC_TALLY_ATOMIC_F32> {
+// SINGLE: every segment is its own run (unsorted stream), so a deterministic
+// partial is one FP32 tally and is stored as FP32 without loss.
+template <int TALLY, bool SINGLE> struct Acc;
+template <bool SINGLE> struct Acc<SMOC_TALLY_ATOMIC_F32, SINGLE> {
     float s = 0.f;
     __device__ __forceinline__ void add(float x) { s += x; }
     __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
@@ -252,7 +257,7 @@ template <> struct Acc<SMOC_TALLY_ATOMIC_F32> {
         s = 0.f;
     }
 };
-template <> struct Acc<SMOC_TALLY_ATOMIC_F64> {
+template <bool SINGLE> struct Acc<SMOC_TALLY_ATOMIC_F64, SINGLE> {
     double s = 0.0;
     __device__ __forceinline__ void add(float x) { s += (double)x; }
     __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
@@ -261,7 +266,7 @@ template <> struct Acc<SMOC_TALLY_ATOMIC_F64> {
         s = 0.0;
     }
 };
-template <> struct Acc<SMOC_TALLY_DET_F64> {
+template <> struct Acc<SMOC_TALLY_DET_F64, false> {
     double s = 0.0;
     __device__ __forceinline__ void add(float x) { s += (double)x; }
     __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
@@ -270,7 +275,7 @@ template <> struct Acc<SMOC_TALLY_DET_F64> {
         s = 0.0;
     }
 };
-template <> struct Acc<SMOC_TALLY_DET_KAHAN> {
+template <> struct Acc<SMOC_TALLY_DET_KAHAN, false> {
     float s = 0.f, c = 0.f;
     __device__ __forceinline__ void add(float x)
     {
@@ -287,6 +292,16 @@ template <> struct Acc<SMOC_TALLY_DET_KAHAN> {
         c = 0.f;
     }
 };
+template <int TALLY> struct AccSingle {
+    float s = 0.f;
+    __device__ __forceinline__ void add(float x) { s = x; }
+    __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
+    {
+        static_cast<float *>(part)[run * G + g] = s;
+    }
+};
+template <> struct Acc<SMOC_TALLY_DET_F64, true> : AccSingle<SMOC_TALLY_DET_F64> {};
+template <> struct Acc<SMOC_TALLY_DET_KAHAN, true> : AccSingle<SMOC_TALLY_DET_KAHAN> {};
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
 {
@@ -304,7 +319,15 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 
 // The segment-stream pointers reach sweep_item as separate restrict kernel
 // parameters so that their wave-uniform loads compile to scalar loads.
-template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+//
+// MODE_SWEEP: attenuate and tally.  MODE_XFER: the affine transfer of the
+// item's segments, psi_out = M psi_in + B per group (kernel.c:289-332 is
+// affine in psi), computed from psi_in = 0 and a running product of 1 - ev;
+// no tally.  A scan over a track's chunks turns the transfers into every
+// chunk's incoming psi.
+enum { MODE_SWEEP = 0, MODE_XFER = 1 };
+
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                            const float *__restrict__ len,
                                            const int32_t *__restrict__ region,
@@ -315,36 +338,39 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 {
     using raw_t = typename Tup<STORE>::raw;
     const int F = a.F;
-    const int64_t it = a.slices == 1 ? item : item / a.slices;
-    const int sl = (int)(item - it * a.slices);
+    const int64_t w = a.slices == 1 ? item : item / a.slices;
+    const int sl = (int)(item - w * a.slices);
+    const int64_t it = a.item_list ? uniform64(a.item_list[w]) : w;
 
-    int64_t t, kb, ke;
+    int64_t t, kb, ke, run = 0;
     if (items) {
-        t = uniform64(items[3 * it]);
-        kb = uniform64(items[3 * it + 1]);
-        ke = uniform64(items[3 * it + 2]);
+        t = uniform64(items[4 * it]);
+        kb = uniform64(items[4 * it + 1]);
+        ke = uniform64(items[4 * it + 2]);
+        run = uniform64(items[4 * it + 3]);
     } else {
         t = it;
         kb = uniform64(track_off[t]);
         ke = uniform64(track_off[t + 1]);
+        if constexpr (TALLY == SMOC_TALLY_DET_F64 || TALLY == SMOC_TALLY_DET_KAHAN)
+            run = uniform64(a.track_run_off[t]);
     }
 
-    // Device rows are padded to Gp >= slices * 64 * GPL groups, so every lane
-    // addresses a valid element; lanes past G compute on padding and write
-    // only padding.
-    const int Gp = a.Gp;
-    int g[GPL];
-    float psi[GPL];
-    Acc<TALLY> acc[GPL];
+    // Device rows are padded to Gp >= slices * 64 * GPL groups.  Lanes past G
+    // load the last valid group's data (same cache lines, no extra traffic),
+    // compute on it and never flush; their psi stores land in padding.
+    const int Gp = a.Gp, G = a.G;
+    int g[GPL], gi[GPL];
+    float psi[GPL], mprod[GPL];
+    Acc<TALLY, !RUNS> acc[GPL];
     const float *pin = a.chunk_psi ? a.chunk_psi + it * Gp : a.psi_in + t * Gp;
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         g[j] = sl * 64 * GPL + j * 64 + lane;
-        psi[j] = pin[g[j]];
+        gi[j] = g[j] < G ? g[j] : G - 1;
+        psi[j] = MODE == MODE_XFER ? 0.f : pin[gi[j]];
+        mprod[j] = 1.f;
     }
-    int64_t run = 0;
-    if constexpr (TALLY == SMOC_TALLY_DET_F64 || TALLY == SMOC_TALLY_DET_KAHAN)
-        run = uniform64(a.track_run_off[t]);
 
     const raw_t *__restrict__ tup = static_cast<const raw_t *>(a.tuples);
     // Software pipeline over the item's segments: while segment k is
@@ -360,7 +386,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         const int64_t row = sg.row >= 0 ? sg.row : 0;
 #pragma unroll
         for (int j = 0; j < GPL; j++)
-            buf[j] = tup[row * Gp + g[j]];
+            buf[j] = tup[row * Gp + gi[j]];
     };
     auto raw_at = [&](int64_t k) {
         return k < ke ? load_raw(len, region, z, k) : RawSeg{0.f, 0.f, -1};
@@ -370,18 +396,26 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 #pragma unroll
         for (int j = 0; j < GPL; j++) {
             const float4 r = unpack<STORE>(buf[j]);
-            float q0, q1, q2;
+            float q0, q1, q2, m;
             fit<STORE>(r, cur.f, F, cur.zin, zin2, q0, q1, q2);
-            acc[j].add(attenuate<EXPM>(r.x, q0, q1, q2, cur.ds, psi[j], tb));
+            const float tally = attenuate<EXPM>(r.x, q0, q1, q2, cur.ds, psi[j], m, tb);
+            if constexpr (MODE == MODE_XFER)
+                mprod[j] *= m;
+            else
+                acc[j].add(tally);
         }
-        // RUNS: the stream is sorted enough that a track stays in one
-        // (region, interval) for several segments; sum the run in registers
-        // and flush when it ends.  Otherwise every segment is its own run.
-        if (!RUNS || nxt.row != cur.row) {
+        if constexpr (MODE == MODE_SWEEP) {
+            // RUNS: the stream is sorted enough that a track stays in one
+            // (region, interval) for several segments; sum the run in
+            // registers and flush when it ends.  Otherwise every segment is
+            // its own run.
+            if (!RUNS || nxt.row != cur.row) {
 #pragma unroll
-            for (int j = 0; j < GPL; j++)
-                acc[j].flush(a.flux, cur.row, run, Gp, g[j]);
-            ++run;
+                for (int j = 0; j < GPL; j++)
+                    if (g[j] < G)
+                        acc[j].flush(a.flux, cur.row, run, Gp, g[j]);
+                ++run;
+            }
         }
     };
     auto dec = [&](const RawSeg &r) { return r.region >= 0 ? decode(r, F) : past_end(); };
@@ -411,8 +445,15 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
             s1 = s3;
         }
     }
-    // chunks other than a track's last carry no psi_out
-    if (!items || ke == uniform64(track_off[t + 1])) {
+    if constexpr (MODE == MODE_XFER) {
+        float *xb = a.xfer + (2 * it) * Gp, *xm = a.xfer + (2 * it + 1) * Gp;
+#pragma unroll
+        for (int j = 0; j < GPL; j++) {
+            xb[g[j]] = psi[j];
+            xm[g[j]] = mprod[j];
+        }
+    } else if (!items || ke == uniform64(track_off[t + 1])) {
+        // chunks other than a track's last carry no psi_out
         float *pout = a.psi_out + t * Gp;
 #pragma unroll
         for (int j = 0; j < GPL; j++)
@@ -438,7 +479,7 @@ __device__ __forceinline__ ExpTab stage_table(const smoc_sweep_args &a, float *s
 }
 
 // One wave per work item, XCD-aware block order.
-template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __global__ void __launch_bounds__(1024)
     k_sweep(smoc_sweep_args a, const float *__restrict__ len, const int32_t *__restrict__ region,
             const float *__restrict__ z, const int64_t *__restrict__ track_off,
@@ -452,12 +493,13 @@ __global__ void __launch_bounds__(1024)
     const int64_t item = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * wpb + wave;
     if (item >= a.n_items * a.slices)
         return;
-    sweep_item<STORE, GPL, EXPM, TALLY, RUNS>(a, len, region, z, track_off, items, item, lane, tb);
+    sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items, item,
+                                                    lane, tb);
 }
 
 // Persistent variant: each wave pulls work items from a device counter until
 // none remain; every wave exits after at most n_work + 1 pulls.
-template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __global__ void __launch_bounds__(1024)
     k_sweep_persistent(smoc_sweep_args a, const float *__restrict__ len,
                        const int32_t *__restrict__ region, const float *__restrict__ z,
@@ -474,7 +516,34 @@ __global__ void __launch_bounds__(1024)
         const int64_t item = uniform64((int64_t)__shfl(got, 0));
         if (item >= n_work)
             break;
-        sweep_item<STORE, GPL, EXPM, TALLY, RUNS>(a, len, region, z, track_off, items, item, lane, tb);
+        sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
+                                                        item, lane, tb);
+    }
+}
+
+// Scan of chunk transfers along each track: chunk_psi[first chunk] = psi_in,
+// chunk_psi[c+1] = B_c + M_c chunk_psi[c].  One wave per (track, slice).
+template <int GPL>
+__global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= a.T * a.slices)
+        return;
+    const int64_t t = item / a.slices;
+    const int sl = (int)(item - t * a.slices);
+    const int Gp = a.Gp;
+    const int64_t cb = a.track_item_off[t], ce = a.track_item_off[t + 1];
+#pragma unroll
+    for (int j = 0; j < GPL; j++) {
+        const int g = sl * 64 * GPL + j * 64 + lane;
+        float psi = a.psi_in[t * Gp + g];
+        for (int64_t c = cb; c < ce; ++c) {
+            a.chunk_psi[c * Gp + g] = psi;
+            if (c + 1 < ce)
+                psi = a.xfer[(2 * c) * Gp + g] + a.xfer[(2 * c + 1) * Gp + g] * psi;
+        }
     }
 }
 
@@ -534,6 +603,14 @@ __global__ void __launch_bounds__(256) k_reset(T *flux, const float *init, int64
         flux[i] = init ? (T)init[i] : (T)0;
 }
 
+__global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
+                                               float4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,7 +621,7 @@ __global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
 // ---- deterministic segmented reduction over run partials -------------------------
 // One wave per ((region, interval), group slice); the runs of a (region,
 // interval) are summed in ascending run-id order, whatever the launch shape.
-template <int TALLY, int GPL>
+template <int TALLY, int GPL, bool SINGLE>
 __global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
 {
     const int lane = threadIdx.x & 63;
@@ -557,58 +634,76 @@ __global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
     const int sl = (int)(item - i * a.slices);
     const int Gp = a.Gp;
     const int64_t rb = a.fsr_run_off[i], re = a.fsr_run_off[i + 1];
+    const int64_t *__restrict__ idx = a.fsr_run_idx;
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         const int g = sl * 64 * GPL + j * 64 + lane;
         const int64_t o = i * Gp + g;
         if constexpr (TALLY == SMOC_TALLY_DET_F64) {
-            const double *part = static_cast<const double *>(a.part);
+            using part_t = typename std::conditional<SINGLE, float, double>::type;
+            const part_t *part = static_cast<const part_t *>(a.part);
             double s = a.init ? (double)a.init[o] : 0.0;
             int64_t r = rb;
             for (; r + 4 <= re; r += 4) {
-                const double v0 = part[a.fsr_run_idx[r] * Gp + g];
-                const double v1 = part[a.fsr_run_idx[r + 1] * Gp + g];
-                const double v2 = part[a.fsr_run_idx[r + 2] * Gp + g];
-                const double v3 = part[a.fsr_run_idx[r + 3] * Gp + g];
+                const double v0 = part[idx[r] * Gp + g];
+                const double v1 = part[idx[r + 1] * Gp + g];
+                const double v2 = part[idx[r + 2] * Gp + g];
+                const double v3 = part[idx[r + 3] * Gp + g];
                 s += v0;
                 s += v1;
                 s += v2;
                 s += v3;
             }
             for (; r < re; ++r)
-                s += part[a.fsr_run_idx[r] * Gp + g];
+                s += (double)part[idx[r] * Gp + g];
             static_cast<double *>(a.flux)[o] = s;
         } else {
 #pragma clang fp contract(off)
-            const float2 *part = static_cast<const float2 *>(a.part);
             float s = a.init ? a.init[o] : 0.f, c = 0.f;
             for (int64_t r = rb; r < re; ++r) {
-                const float2 p = part[a.fsr_run_idx[r] * Gp + g];
-                float y = p.x - c;
+                float px, pc;
+                if constexpr (SINGLE) {
+                    px = static_cast<const float *>(a.part)[idx[r] * Gp + g];
+                    pc = 0.f;
+                } else {
+                    const float2 p = static_cast<const float2 *>(a.part)[idx[r] * Gp + g];
+                    px = p.x;
+                    pc = p.y;
+                }
+                float y = px - c;
                 float t = s + y;
                 c = (t - s) - y;
                 s = t;
-                y = -p.y - c;
-                t = s + y;
-                c = (t - s) - y;
-                s = t;
+                if constexpr (!SINGLE) {
+                    y = -pc - c;
+                    t = s + y;
+                    c = (t - s) - y;
+                    s = t;
+                }
             }
             static_cast<float *>(a.flux)[o] = s - c;
         }
     }
 }
 
-template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
-int launch_sweep_r(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
+int launch_sweep_m(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 {
     if (a->queue)
-        hipLaunchKernelGGL((k_sweep_persistent<STORE, GPL, EXPM, TALLY, RUNS>), dim3(grid),
+        hipLaunchKernelGGL((k_sweep_persistent<STORE, GPL, EXPM, TALLY, RUNS, MODE>), dim3(grid),
                            dim3(64 * wpb), 0, s, *a, a->len, a->region, a->z, a->track_off,
                            a->items);
     else
-        hipLaunchKernelGGL((k_sweep<STORE, GPL, EXPM, TALLY, RUNS>), dim3(grid), dim3(64 * wpb),
-                           0, s, *a, a->len, a->region, a->z, a->track_off, a->items);
+        hipLaunchKernelGGL((k_sweep<STORE, GPL, EXPM, TALLY, RUNS, MODE>), dim3(grid),
+                           dim3(64 * wpb), 0, s, *a, a->len, a->region, a->z, a->track_off,
+                           a->items);
     return (int)hipGetLastError();
+}
+
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
+int launch_sweep_r(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
+{
+    return launch_sweep_m<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, wpb, grid, s);
 }
 
 template <int STORE, int GPL, int EXPM, int TALLY>
@@ -622,6 +717,9 @@ int launch_sweep_t(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 template <int STORE, int GPL, int EXPM>
 int launch_sweep_tally(const smoc_sweep_args *a, int tally, int wpb, int grid, hipStream_t s)
 {
+    if (a->xfer_mode)  // no tally in the transfer pass
+        return launch_sweep_m<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F32, false, MODE_XFER>(a, wpb,
+                                                                                    grid, s);
     switch (tally) {
     case SMOC_TALLY_ATOMIC_F32:
         return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F32>(a, wpb, grid, s);
@@ -697,6 +795,15 @@ extern "C" int smoc_launch_reset(void *flux, const float *init, int64_t n, int f
     return (int)hipGetLastError();
 }
 
+extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream)
+{
+    if (n16 == 0)
+        return 0;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    return (int)hipGetLastError();
+}
+
 extern "C" int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream)
 {
     if (n == 0)
@@ -720,6 +827,30 @@ extern "C" int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl,
     return (int)hipErrorInvalidValue;
 }
 
+template <int TALLY, int GPL>
+int launch_reduce_t(const smoc_reduce_args *a, unsigned nb, dim3 blk, hipStream_t s)
+{
+    if (a->single)
+        hipLaunchKernelGGL((k_reduce<TALLY, GPL, true>), dim3(nb), blk, 0, s, *a);
+    else
+        hipLaunchKernelGGL((k_reduce<TALLY, GPL, false>), dim3(nb), blk, 0, s, *a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream)
+{
+    const int64_t n = a->T * a->slices;
+    if (n == 0)
+        return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = blocks_for(n, 4);
+    if (gpl == 2)
+        hipLaunchKernelGGL(k_scan<2>, dim3(nb), dim3(256), 0, s, *a);
+    else
+        hipLaunchKernelGGL(k_scan<1>, dim3(nb), dim3(256), 0, s, *a);
+    return (int)hipGetLastError();
+}
+
 extern "C" int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int wpb,
                                   void *stream)
 {
@@ -729,18 +860,11 @@ extern "C" int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl,
         return 0;
     const unsigned nb = blocks_for(n, wpb);
     const dim3 blk(64 * wpb);
-    if (tally == SMOC_TALLY_DET_F64) {
-        if (gpl == 2)
-            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_F64, 2>), dim3(nb), blk, 0, s, *a);
-        else
-            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_F64, 1>), dim3(nb), blk, 0, s, *a);
-    } else if (tally == SMOC_TALLY_DET_KAHAN) {
-        if (gpl == 2)
-            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_KAHAN, 2>), dim3(nb), blk, 0, s, *a);
-        else
-            hipLaunchKernelGGL((k_reduce<SMOC_TALLY_DET_KAHAN, 1>), dim3(nb), blk, 0, s, *a);
-    } else {
-        return (int)hipErrorInvalidValue;
-    }
-    return (int)hipGetLastError();
+    if (tally == SMOC_TALLY_DET_F64)
+        return gpl == 2 ? launch_reduce_t<SMOC_TALLY_DET_F64, 2>(a, nb, blk, s)
+                        : launch_reduce_t<SMOC_TALLY_DET_F64, 1>(a, nb, blk, s);
+    if (tally == SMOC_TALLY_DET_KAHAN)
+        return gpl == 2 ? launch_reduce_t<SMOC_TALLY_DET_KAHAN, 2>(a, nb, blk, s)
+                        : launch_reduce_t<SMOC_TALLY_DET_KAHAN, 1>(a, nb, blk, s);
+    return (int)hipErrorInvalidValue;
 }

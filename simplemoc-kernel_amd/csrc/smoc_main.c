@@ -27,6 +27,7 @@ typedef struct {
     uint64_t seed;
     int steps;
     int nthreads;
+    int p_given;
     smoc_launch launch;
 } options;
 
@@ -103,8 +104,10 @@ static void read_CLI(int argc, char *argv[], options *o)
             o->in.segments = atoll(NEXT());
         else if (strcmp(arg, "-e") == 0)
             o->in.egroups = atoi(NEXT());
-        else if (strcmp(arg, "-p") == 0)
+        else if (strcmp(arg, "-p") == 0) {
             o->in.seg_per_block = atoi(NEXT());
+            o->p_given = 1;
+        }
         else if (strcmp(arg, "-d") == 0)
             o->device = atoi(NEXT());
         else if (strcmp(arg, "--storage") == 0)
@@ -159,14 +162,17 @@ int main(int argc, char *argv[])
     o.track_len = 100;
     o.seed = 42;
     o.steps = 1;
-    o.exp_mode = SMOC_EXP_NATIVE;
+    o.exp_mode = SMOC_EXP_ACCURATE; /* --exp native: the reference's literal form */
     o.storage = SMOC_STORE_F32;
     o.tally = SMOC_TALLY_ATOMIC_F32;
     read_CLI(argc, argv, &o);
+    smoc_set_host_threads(o.nthreads);
     smoc_derive_regions(&o.in);
     const int R = o.regions_override > 0 ? o.regions_override : o.in.source_3D_regions;
     const int G = o.in.egroups, F = o.in.fine_axial_intervals;
-    o.launch.seg_per_block = o.in.seg_per_block;
+    /* -p: segments per work item (tracks longer than p are split into chunks
+     * joined by the psi scan); without -p tracks are swept whole */
+    o.launch.seg_per_block = o.p_given ? o.in.seg_per_block : 0;
 
     int64_t S = o.in.segments, T = 0;
     int64_t *track_off = NULL;
@@ -213,7 +219,10 @@ int main(int argc, char *argv[])
     printf("%-25s", "Tracks:");
     fancy_int(T);
     printf("%-25s%.2f\n", "Memory Estimate (MB):", mem_mb);
-    printf("%-25s%d\n", "Segments per block:", o.in.seg_per_block);
+    if (o.p_given)
+        printf("%-25s%d\n", "Segments per block:", o.in.seg_per_block);
+    else
+        printf("%-25s%s\n", "Segments per block:", "whole tracks");
     printf("%-25s%s\n", "Exponential Table:", o.exp_mode == SMOC_EXP_TABLE ? "ON" : "OFF");
     printf("%-25s%s\n", "Exponential:",
            (const char *[]){"native", "table", "accurate (series for tau < 1)"}[o.exp_mode]);
@@ -290,8 +299,11 @@ int main(int argc, char *argv[])
     printf("%-25s%.6f seconds\n", "Runtime:", mean_s);
     printf("%-25s%.6f ns\n", "Time per Intersection:", tpi);
     printf("%-25s%.4e\n", "Attenuations/s:", (double)S * G / mean_s);
-    printf("%-25s%.3f ms (sweep %.3f, reduce %.3f)\n", "Best run:", best.total_ms, best.sweep_ms,
-           best.reduce_ms);
+    printf("%-25s%.3f ms (transfer+scan %.3f, sweep %.3f, reduce %.3f)\n", "Best run:",
+           best.total_ms, best.xfer_ms, best.sweep_ms, best.reduce_ms);
+    printf("%-25sgpl %d, wpb %d, chunk %d, persistent %d, graph %d\n", "Launch:",
+           best.used.groups_per_lane, best.used.waves_per_block, best.used.chunk,
+           best.used.persistent, best.used.use_graph);
     printf("%-25s%.17g\n", "fine_flux checksum:", ck_flux);
     printf("%-25s%.17g\n", "state_flux checksum:", ck_psi);
     border_print();

@@ -56,7 +56,8 @@ class Launch(ctypes.Structure):
 
 class Timing(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("reset_ms", ctypes.c_double),
-                ("sweep_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double),
+                ("xfer_ms", ctypes.c_double), ("sweep_ms", ctypes.c_double),
+                ("reduce_ms", ctypes.c_double),
                 ("segments", ctypes.c_int64), ("work_items", ctypes.c_int64),
                 ("used", Launch)]
 
@@ -92,10 +93,12 @@ SIGNATURES = {
     "smoc_tracks_fixed": (_i, [_i64, _i, _I64, ctypes.POINTER(_i64)]),
     "smoc_tracks_lognormal": (_i, [_u64, _i64, _d, _d, _i, _i, _I64, ctypes.POINTER(_i64)]),
     "smoc_sort_segments": (_i, [_i64, _i, _i, _F32, _I32, _F32]),
+    "smoc_set_host_threads": (_i, [_i]),
     "smoc_checksum_f64": (_d, [_F64, _i64]),
     "smoc_checksum_f32": (_d, [_F32, _i64]),
     "smoc_device_count": (_i, [ctypes.POINTER(_i)]),
     "smoc_device_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i)]),
+    "smoc_probe_copy": (_i, [_i, _i64, _i, ctypes.POINTER(_d)]),
     "smoc_create": (_i, [ctypes.POINTER(_P), ctypes.POINTER(Params)]),
     "smoc_destroy": (_i, [_P]),
     "smoc_set_regions": (_i, [_P, _F32, _F32, _P]),
@@ -162,6 +165,13 @@ def device_info(device: int = 0):
     return name.value.decode(), cus.value
 
 
+def probe_copy(nbytes: int, reps: int = 5, device: int = 0) -> float:
+    """Mean time (ms) of the plain copy kernel over nbytes (read + write)."""
+    ms = _d()
+    _check(lib().smoc_probe_copy(device, nbytes, reps, ctypes.byref(ms)), "smoc_probe_copy")
+    return ms.value
+
+
 @dataclass
 class Problem:
     """One attenuation problem: region data, segment stream and tracks."""
@@ -225,7 +235,7 @@ def generate(S: int, G: int = 100, R: int = 6750, F: int = 5, seed: int = 42,
 class Context:
     """A device context: smoc_create ... smoc_destroy."""
 
-    def __init__(self, G: int, R: int, F: int = 5, storage="f32", exp="native",
+    def __init__(self, G: int, R: int, F: int = 5, storage="f32", exp="accurate",
                  tally="atomic-f32", device: int = 0):
         self.G, self.R, self.F = G, R, F
         st = STORAGE[storage] if isinstance(storage, str) else storage

@@ -1,0 +1,66 @@
+"""The smoc driver keeps the reference command line (src/cuda/io.cu:107-162):
+-s segments, -e groups, -p segments per block, -d device; unknown or
+incomplete options print the usage and exit(1)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+import smoc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run(*args, timeout=120):
+    return subprocess.run([smoc.DRIVER_PATH, *args], capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def test_unknown_option_prints_usage():
+    r = run("-x")
+    assert r.returncode == 1
+    assert "Usage:" in r.stdout and "-s <segments>" in r.stdout and "-p <segs per block>" in r.stdout
+
+
+@pytest.mark.parametrize("args", [("-s",), ("-e",), ("-p", "0"), ("--tally", "bogus"),
+                                  ("-s", "-5")])
+def test_bad_values_exit_1(args):
+    assert run(*args).returncode == 1
+
+
+def test_no_device_is_an_error():
+    if smoc.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    r = run("-s", "1000", "-e", "8")
+    assert r.returncode == 2
+    assert "no usable HIP device" in r.stderr
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_report_and_checksums_match_api(gpu_ready):
+    r = run("-s", "200000", "-e", "100", "-p", "512", "--tally", "det-f64", "--exp", "accurate",
+            "--steps", "2")
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    for line in ("INPUT SUMMARY", "3D Source Regions:       6750", "Segments:                200,000",
+                 "Runtime:", "Time per Intersection:", "fine_flux checksum:",
+                 "state_flux checksum:"):
+        assert line in out, line
+    ck = {l.split(":")[0]: float(l.split(":")[1]) for l in out.splitlines()
+          if "checksum:" in l}
+    prob = smoc.generate(200_000, G=100, R=6750, F=5, seed=42)
+    with smoc.Context(100, 6750, 5, exp="accurate", tally="det-f64") as ctx:
+        ctx.load(prob)
+        ctx.run()
+        flux, psi = ctx.results()
+    f, p = smoc.checksums(flux, psi)
+    # deterministic tally: the driver and the API produce identical sums
+    assert ck["fine_flux checksum"] == float(f"{f:.17g}")
+    assert ck["state_flux checksum"] == float(f"{p:.17g}")

@@ -1,0 +1,119 @@
+"""Batched sweeps: tracks of uneven length split into chunks joined by the
+affine psi scan, persistent work-queue kernels, hipGraph replay and the
+autotuner -- each against whole-track sweeps and the FP64 oracle."""
+import numpy as np
+import pytest
+
+import pyoracle
+import smoc
+from conftest import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ragged():
+    # lognormal batches, shorter than the benchmark's so the oracle is quick
+    return smoc.generate(0, G=100, R=1000, F=5, seed=21, lognormal_tracks=48, median=600.0,
+                         sigma=1.0, lo=16, hi=8000)
+
+
+@pytest.fixture(scope="module")
+def ragged_oracle(ragged):
+    return pyoracle.sweep_f64(ragged, pyoracle.EXP_NATIVE, nthreads=8)
+
+
+def sweep(prob, tally="atomic-f32", storage="f32", exp="accurate", ctx_out=None, **launch):
+    with smoc.Context(prob.G, prob.R, prob.F, storage=storage, exp=exp, tally=tally) as ctx:
+        ctx.load(prob)
+        tm = ctx.run(**launch)
+        flux, psi = ctx.results()
+    return flux, psi, tm
+
+
+@pytest.mark.parametrize("chunk", [0, 64, 500, 2048])
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("tally", ["atomic-f32", "det-f64"])
+def test_chunked_vs_oracle(gpu_ready, ragged, ragged_oracle, chunk, persistent, tally):
+    print(f"case chunk={chunk} persistent={persistent} {tally} S={ragged.S}", flush=True)
+    flux, psi, tm = sweep(ragged, tally, chunk=chunk, persistent=persistent)
+    of, op = ragged_oracle
+    ef = abs(flux.sum() - of.sum()) / np.abs(of).sum()
+    print(f"  l1-scaled fine_flux {ef:.2e}, normwise {normwise(flux, of):.2e} / "
+          f"{normwise(psi, op):.2e}; used chunk {tm.used.chunk}", flush=True)
+    assert tm.used.chunk == chunk
+    assert ef < 1e-5 and normwise(flux, of) < 1e-5 and normwise(psi, op) < 1e-5
+
+
+def test_chunking_changes_only_rounding(gpu_ready, ragged):
+    _, whole, _ = sweep(ragged, "det-f64")
+    _, split, _ = sweep(ragged, "det-f64", chunk=100)
+    assert normwise(split, whole) < 2e-6
+
+
+def test_deterministic_chunked_bit_exact(gpu_ready, ragged):
+    with smoc.Context(ragged.G, ragged.R, ragged.F, tally="det-f64") as ctx:
+        ctx.load(ragged)
+        ctx.run(chunk=256, persistent=1)
+        f0, p0 = ctx.results()
+        for _ in range(3):
+            ctx.run(chunk=256, persistent=1)
+            f1, p1 = ctx.results()
+            assert np.array_equal(f0, f1) and np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("launch", [{"chunk": 300, "persistent": 1}, {"chunk": 0},
+                                    {"chunk": 128, "persistent": 0, "groups_per_lane": 1}])
+def test_graph_replays_match_eager(gpu_ready, ragged, launch):
+    # every replay is checked against the eager run, with eager work and
+    # synchronisation between replays
+    with smoc.Context(ragged.G, ragged.R, ragged.F, tally="det-f64") as ctx:
+        ctx.load(ragged)
+        ctx.run(**launch)
+        fe, pe = ctx.results()
+        for _ in range(4):
+            ctx.run(use_graph=1, **launch)
+            fg, pg = ctx.results()
+            assert np.array_equal(fg, fe) and np.array_equal(pg, pe)
+            ctx.run(**launch)
+            f2, p2 = ctx.results()
+            assert np.array_equal(f2, fe) and np.array_equal(p2, pe)
+
+
+def test_autotune_picks_valid_launch(gpu_ready, ragged, ragged_oracle):
+    with smoc.Context(ragged.G, ragged.R, ragged.F, exp="accurate", tally="atomic-f32") as ctx:
+        ctx.load(ragged)
+        tm = ctx.run(autotune=1)
+        flux, psi = ctx.results()
+        u = tm.used
+        print(f"  tuned: gpl {u.groups_per_lane} wpb {u.waves_per_block} chunk {u.chunk} "
+              f"persistent {u.persistent}", flush=True)
+        assert u.groups_per_lane in (1, 2) and 1 <= u.waves_per_block <= 16
+        tm2 = ctx.run(autotune=1)  # cached choice
+        assert (tm2.used.chunk, tm2.used.persistent) == (u.chunk, u.persistent)
+    of, op = ragged_oracle
+    assert normwise(flux, of) < 1e-5 and normwise(psi, op) < 1e-5
+
+
+def test_sorted_det_refuses_chunks(gpu_ready):
+    prob = smoc.generate(20_000, G=64, R=50, F=5, seed=2, sorted_stream=True, track_len=2000)
+    with smoc.Context(64, 50, 5, tally="det-f64") as ctx:
+        ctx.load(prob)
+        with pytest.raises(smoc.SmocError) as e:
+            ctx.run(chunk=100)
+        assert e.value.code == smoc.ERR_ARG
+        ctx.run(chunk=100 * 0)  # whole tracks work
+
+
+@pytest.mark.parametrize("gpl", [1, 2])
+@pytest.mark.parametrize("wpb", [1, 4])
+@pytest.mark.parametrize("chunk", [0, 1024])
+@pytest.mark.parametrize("tally", ["atomic-f32", "det-f64"])
+def test_launch_shapes_vs_oracle(gpu_ready, ragged, ragged_oracle, gpl, wpb, chunk, tally):
+    flux, psi, tm = sweep(ragged, tally, groups_per_lane=gpl, waves_per_block=wpb, chunk=chunk)
+    of, op = ragged_oracle
+    ef, ep = normwise(flux, of), normwise(psi, op)
+    bad = np.unravel_index(np.argmax(np.abs(flux - of)), flux.shape)
+    print(f"  gpl {gpl} wpb {wpb} chunk {chunk} {tally}: normwise {ef:.2e} / {ep:.2e}; "
+          f"worst flux entry {bad}", flush=True)
+    assert ef < 1e-5 and ep < 1e-5
