@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the segment-attenuation sweep on one MI355X.
 
-Workload (default): 50,000,000 segments x 100 energy groups, 6750 source
-regions x 5 fine axial intervals, tracks of 100 segments, FP32, atomic
-fine_flux tally -- the plain kernel at the reference's default scale.  A
-step is one smoc_run over the whole stream (fine_flux reset, sweep); inputs
-are resident in HBM before timing starts.
+Workload (the metric's configuration): 50,000,000 segments x 100 energy
+groups, 6750 source regions x 5 fine axial intervals, tracks of 100
+segments, FP32 storage and arithmetic, atomic FP32 fine_flux tally -- the
+plain kernel at the reference's default scale.  A step is one smoc_run over
+the whole stream (fine_flux reset + sweep); inputs are resident in HBM
+before timing starts.  `value` is segment-group attenuations/s of that step.
 
-Prints one JSON line with the metric (segment-group attenuations/s), the
-roofline of the sweep kernel (algorithmic bytes per segment-group / measured
-kernel time, HIP events on the library's stream) and the reference's own CPU
-attenuation code timed on this host's cores on a bounded sample of the same
-workload.
+Also measured, same shape:
+  * `mixed`: the batched mixed-precision path -- the stream pre-sorted by
+    (region, fine interval), BF16 storage of sigT / fine_source, FP32 math,
+    runs summed in registers and a deterministic FP64 segmented reduction;
+  * `roofline`: algorithmic bytes per segment-group of the dominant kernel
+    (k_sweep) over its mean duration from HIP events on the library stream;
+    `traffic` comes from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary committed under profiles/ for this workload;
+  * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
+    on this host's cores on a bounded sample of the same stream, plus the
+    accuracy of both GPU paths against the FP64 oracle (the checker).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,12 +35,14 @@ import numpy as np  # noqa: E402
 import smoc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_plain_50Mx100_traffic.txt")
 
 
 def alg_bytes_per_sg(storage: str, tally: str, G: int, L: float) -> float:
-    """Reference-formulation bytes per segment-group: sigT + 3 source values
-    + tally read and write + segment stream (12 B / G) + psi in/out (8 B /
-    (G L) per group... i.e. 8 / L per segment-group)."""
+    """Reference-formulation bytes per segment-group: sigT + the 3-point source
+    stencil + tally read and write + the segment stream (12 B per segment,
+    shared by G groups) + psi in and out (8 B per track-group, shared by the
+    L segments of a track)."""
     b = 4 if storage == "f32" else 2
     bt = {"atomic-f32": 4, "atomic-f64": 8, "det-f64": 8, "det-kahan": 8}[tally]
     return b + 3 * b + 2 * bt + 12.0 / G + 8.0 / L
@@ -51,7 +61,9 @@ def parse():
     ap.add_argument("--tally", default="atomic-f32")
     ap.add_argument("--gpl", type=int, default=0)
     ap.add_argument("--wpb", type=int, default=0)
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-sample-segments", type=int, default=4_000_000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (counter "
@@ -59,10 +71,33 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(prob, sample_segments: int):
-    """The reference's attenuate_segment (oracle/_ref, built from
-    src/cpu/kernel.c + init.c with the reference's GNU flags) over the first
-    tracks of the same stream, OpenMP over tracks on this host's cores."""
+def timed_steps(ctx, steps, warmup, launch):
+    for _ in range(warmup):
+        ctx.run(**launch)
+    sweeps, totals = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tm = ctx.run(**launch)  # returns after the run's last event has completed
+        sweeps.append(tm.sweep_ms)
+        totals.append(tm.total_ms)
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean(sweeps)), float(np.mean(totals)), tm.used.as_dict()
+
+
+def measured_traffic():
+    """HBM bytes per k_sweep launch from the committed rocprofv3 summary:
+    FETCH_SIZE x its copy-kernel calibration factor + WRITE_SIZE x its."""
+    try:
+        txt = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_leg(prob, sample_segments, gpu_plain, mixed, accuracy):
+    """Reference CPU baseline on a bounded sample, and the checker: FP64
+    oracle on the full stream for the accuracy of the GPU results."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
@@ -77,26 +112,54 @@ def cpu_baseline(prob, sample_segments: int):
         ref = pyoracle.Ref("exp")
         ref.init(prob.G, prob.R, prob.F)
         ref.set_regions(prob.sigT, prob.fine_source)
-        sec, _ = ref.sweep(prob.track_off[: T + 1], prob.region[:S], fai, prob.psi_in[:T], threads)
-        kind = "reference"
+        sec, _ = ref.sweep(prob.track_off[: T + 1], prob.region[:S], fai, prob.psi_in[:T],
+                           threads)
+        kind, what = "reference", ("the reference's attenuate_segment (src/cpu/kernel.c, "
+                                   "-Ofast -fopenmp, dynamic schedule over tracks, per-(region, "
+                                   "interval) locks); ds and zin fixed at its constants")
     else:
         sub = smoc.Problem(prob.G, prob.R, prob.F, prob.sigT, prob.fine_source, prob.len[:S],
                            prob.region[:S], prob.z[:S], prob.track_off[: T + 1], prob.psi_in[:T])
         sec, _, _ = pyoracle.baseline_f32(sub, pyoracle.EXP_NATIVE, threads)
-        kind = "port"
-    rate = S * prob.G / sec
-    return {"value": rate, "unit": "segment-group attenuations/s", "cores": threads,
+        kind, what = "port", "the C restatement of attenuate_segment, OpenMP over tracks"
+    base = {"value": S * prob.G / sec, "unit": "segment-group attenuations/s", "cores": threads,
             "kind": kind,
-            "sample": f"first {T} tracks ({S} segments x {prob.G} groups) of the same stream; "
-                      f"{'reference attenuate_segment (src/cpu/kernel.c, -Ofast -fopenmp), ds/zin fixed at its constants' if kind == 'reference' else 'C restatement, OpenMP'}; "
-                      f"{sec:.3f} s wall"}
+            "sample": f"first {T} tracks ({S} segments x {prob.G} groups) of the benchmark "
+                      f"stream, {what}; {sec:.3f} s wall"}
+    acc = None
+    if accuracy:
+        acc = {}
+        t0 = time.time()
+        of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=threads)
+        f, p = gpu_plain
+        acc["plain_vs_fp64"] = {
+            "fine_flux_checksum_rel": abs(f.sum() - of.sum()) / abs(of.sum()),
+            "state_flux_checksum_rel": abs(p.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+        if mixed is not None:
+            mprob, mf, mp, storage = mixed
+            of, op = pyoracle.sweep_f64(mprob, pyoracle.EXP_NATIVE, nthreads=threads)
+            acc["mixed_vs_fp64_on_fp32_masters"] = {
+                "fine_flux_checksum_rel": abs(mf.sum() - of.sum()) / abs(of.sum()),
+                "state_flux_checksum_rel":
+                    abs(mp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from conftest import quantized
+            of, op = pyoracle.sweep_f64(quantized(mprob, storage), pyoracle.EXP_NATIVE,
+                                        nthreads=threads)
+            acc["mixed_vs_fp64_on_dequantised_inputs"] = {
+                "fine_flux_checksum_rel": abs(mf.sum() - of.sum()) / abs(of.sum()),
+                "state_flux_checksum_rel":
+                    abs(mp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+        acc["oracle_s"] = time.time() - t0
+    return base, acc
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                          "(the sweep shards over tracks, see DESIGN.md)"}))
+                          "(the sweep shards over tracks; the multi-GPU design is in "
+                          "DESIGN.md)"}))
         sys.exit(2)
     if args.copy_probe_mib:
         nbytes = args.copy_probe_mib << 20
@@ -105,8 +168,9 @@ def main():
                           "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
         return
     G, S, L = args.groups, args.segments, args.track_len
+    R, F = 6750, 5
     t0 = time.time()
-    prob = smoc.generate(S, G=G, R=6750, F=5, seed=42, track_len=L)
+    prob = smoc.generate(S, G=G, R=R, F=F, seed=42, track_len=L)
     gen_s = time.time() - t0
     dev_name, cus = smoc.device_info(0)
     launch = {}
@@ -115,28 +179,18 @@ def main():
     if args.wpb:
         launch["waves_per_block"] = args.wpb
 
-    with smoc.Context(G, prob.R, prob.F, storage=args.storage, exp=args.exp,
-                      tally=args.tally) as ctx:
+    with smoc.Context(G, R, F, storage=args.storage, exp=args.exp, tally=args.tally) as ctx:
         ctx.load(prob)
-        for _ in range(args.warmup):
-            ctx.run(**launch)
-        sweeps, totals = [], []
-        t_start = time.perf_counter()
-        for _ in range(args.steps):
-            tm = ctx.run(**launch)  # returns after the run's last event completes
-            sweeps.append(tm.sweep_ms)
-            totals.append(tm.total_ms)
-        wall = time.perf_counter() - t_start
-        used = tm.used.as_dict()
+        wall, sweep_ms, total_ms, used = timed_steps(ctx, args.steps, args.warmup, launch)
         flux, psi = ctx.results()
     ck_f, ck_p = smoc.checksums(flux, psi)
-
     sg = S * G
-    ms_per_step = wall / args.steps * 1e3
     value = sg / (wall / args.steps)
-    sweep_ms = float(np.mean(sweeps))
     B = alg_bytes_per_sg(args.storage, args.tally, G, L)
     achieved = B * sg / (sweep_ms * 1e-3) / 1e9
+    traffic = measured_traffic() if (args.storage, args.tally, S, G) == \
+        ("f32", "atomic-f32", 50_000_000, 100) else None
+
     out = {
         "metric": "segment-group attenuations/s",
         "value": value,
@@ -144,27 +198,54 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.storage == "f32" else f"{args.storage}-storage/f32-math",
         "data": "synthetic (seeded counter-based generator, seed 42)",
-        "config": {"workload": f"plain sweep: {S} segments x {G} groups, 6750 regions x 5 fine "
-                               f"intervals, tracks of {L} segments",
-                   "segments": S, "groups": G, "regions": 6750, "fine_axial": 5,
+        "config": {"workload": f"plain sweep: {S} segments x {G} energy groups, {R} source "
+                               f"regions x {F} fine axial intervals, tracks of {L} segments "
+                               f"in generation order, atomic fine_flux tally",
+                   "segments": S, "groups": G, "regions": R, "fine_axial": F,
                    "track_len": L, "storage": args.storage, "exp": args.exp,
                    "tally": args.tally, "launch": used},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sweep", "kernel_ms": sweep_ms, "alg_bytes_per_sg": B},
-        "hbm_gbs_alg": achieved,
+        "hbm_gbs": achieved,
         "checksums": {"fine_flux": ck_f, "state_flux": ck_p},
         "device": f"{dev_name}, {cus} CUs",
         "input_gen_s": gen_s,
     }
+
+    mixed = None
+    if not args.no_mixed:
+        mstore, mtally = "bf16", "det-f64"
+        mprob = smoc.generate(S, G=G, R=R, F=F, seed=42, track_len=L, sorted_stream=True)
+        with smoc.Context(G, R, F, storage=mstore, exp=args.exp, tally=mtally) as ctx:
+            ctx.load(mprob)
+            mwall, msweep, mtotal, mused = timed_steps(ctx, args.steps, args.warmup, {})
+            mf, mp = ctx.results()
+        mvalue = sg / (mwall / args.steps)
+        mB = alg_bytes_per_sg(mstore, mtally, G, L)
+        out["mixed"] = {
+            "workload": "same shape; stream pre-sorted by (region, fine interval), tracks of "
+                        f"{L} consecutive sorted segments; {mstore} storage, FP32 math, runs "
+                        "summed in registers, deterministic FP64 segmented reduction",
+            "value": mvalue, "ms_per_step": mwall / args.steps * 1e3,
+            "vs_plain": mvalue / value, "sweep_ms": msweep, "total_ms": mtotal,
+            "hbm_gbs_alg": mB * sg / (mtotal * 1e-3) / 1e9, "alg_bytes_per_sg": mB,
+            "launch": mused,
+            "checksums": dict(zip(("fine_flux", "state_flux"), smoc.checksums(mf, mp)))}
+        mixed = (mprob, mf, mp, mstore)
+
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_sample_segments)
+        base, acc = cpu_leg(prob, args.cpu_sample_segments, (flux, psi), mixed,
+                            not args.no_accuracy)
+        out["cpu_baseline"] = base
+        if acc is not None:
+            out["accuracy"] = acc
     print(json.dumps(out))
 
 

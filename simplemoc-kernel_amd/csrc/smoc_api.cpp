@@ -170,7 +170,8 @@ int smoc_device_info(int device, char *name, int name_cap, int *compute_units)
         return SMOC_ERR_NODEV;
     }
     if (name && name_cap > 0)
-        snprintf(name, (size_t)name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+        snprintf(name, (size_t)name_cap, "%s (%s)", prop.name[0] ? prop.name : "AMD GPU",
+                 prop.gcnArchName);
     if (compute_units)
         *compute_units = prop.multiProcessorCount;
     return SMOC_OK;
@@ -205,7 +206,12 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const int grid = prop.multiProcessorCount * 8;
+    const int64_t nblk = (bytes / 16 + 1023) / 1024;
+    if (nblk > INT32_MAX) {
+        smoc_set_error("smoc_probe_copy: %lld bytes is too large", (long long)bytes);
+        return SMOC_ERR_ARG;
+    }
+    const int grid = (int)nblk;
     LAUNCHCHK(smoc_launch_copy(a.p, b.p, bytes / 16, grid, s));  // warm-up
     HIPCHK(hipEventRecord(e0, s));
     for (int r = 0; r < reps; r++)

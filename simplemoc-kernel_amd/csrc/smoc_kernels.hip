@@ -603,12 +603,20 @@ __global__ void __launch_bounds__(256) k_reset(T *flux, const float *init, int64
         flux[i] = init ? (T)init[i] : (T)0;
 }
 
+// Plain copy, 4 independent 16-byte loads in flight per lane.
 __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
                                                float4 *__restrict__ dst, int64_t n)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        if (base + u * 256 < n)
+            v[u] = src[base + u * 256];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        if (base + u * 256 < n)
+            dst[base + u * 256] = v[u];
 }
 
 __global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
