@@ -1,0 +1,123 @@
+"""The benchmark configurations at their full sizes, through the C ABI.
+
+Configs 1-4 are checked against the FP64 oracle over the whole stream (the
+oracle runs multi-threaded on the host).  Config 5 (2.56e10 segment-groups)
+is checked through size-independent properties: bit-exact determinism of the
+deterministic tally, psi_out of sampled tracks against the FP64 oracle on
+those tracks alone (psi depends only on its own track), and agreement of the
+deterministic FP64 tally with the independent atomic FP64 tally.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+import smoc
+from conftest import normwise, quantized, rel
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def oracle(prob, exp="accurate", storage="f32"):
+    q = quantized(prob, storage)
+    return pyoracle.sweep_f64(q, pyoracle.EXP_TABLE if exp == "table" else pyoracle.EXP_NATIVE,
+                              nthreads=THREADS)
+
+
+def assert_close(flux, psi, of, op, tol=1e-5):
+    e = (rel(flux.sum(), of.sum()), rel(psi.astype(np.float64).sum(), op.sum()),
+         normwise(flux, of), normwise(psi, op))
+    print(f"  checksum rel err fine_flux {e[0]:.2e} state_flux {e[1]:.2e}; normwise "
+          f"{e[2]:.2e} / {e[3]:.2e}", flush=True)
+    assert max(e) <= tol, e
+
+
+def test_config1_reference_table(gpu_ready):
+    prob = smoc.generate(1_000_000, G=100, R=6750, F=5, seed=42)
+    print("case configs[0]: 1e6 x 100, table exp", flush=True)
+    with smoc.Context(100, 6750, 5, exp="table") as ctx:
+        ctx.load(prob)
+        ctx.run()
+        flux, psi = ctx.results()
+    of, op = oracle(prob, "table")
+    e = (rel(flux.sum(), of.sum()), rel(psi.astype(np.float64).sum(), op.sum()))
+    print(f"  table vs FP64-with-table checksums {e[0]:.2e} {e[1]:.2e}", flush=True)
+    assert max(e) <= 1e-5
+
+
+def test_config2_plain_full(gpu_ready):
+    prob = smoc.generate(50_000_000, G=100, R=6750, F=5, seed=42)
+    print("case configs[1]: 5e7 x 100 plain FP32 atomic, p = 512", flush=True)
+    with smoc.Context(100, 6750, 5) as ctx:
+        ctx.load(prob)
+        ctx.run(seg_per_block=512)  # the reference's -p 512 start; tracks of 100 stay whole
+        flux, psi = ctx.results()
+        ctx.run()
+        _, psi2 = ctx.results()
+    assert np.array_equal(psi, psi2)
+    of, op = oracle(prob)
+    assert_close(flux, psi, of, op)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("tally", ["det-kahan", "det-f64"])
+def test_config3_precision(gpu_ready, storage, tally):
+    prob = smoc.generate(1_000_000, G=100, R=6750, F=5, seed=42)
+    print(f"case configs[2]: {storage} storage, {tally}", flush=True)
+    with smoc.Context(100, 6750, 5, storage=storage, tally=tally) as ctx:
+        ctx.load(prob)
+        ctx.run()
+        flux, psi = ctx.results()
+    of, op = oracle(prob, storage=storage)
+    assert_close(flux, psi, of, op)
+    mf, mp = oracle(prob)
+    print(f"  vs FP64 on the FP32 masters: fine_flux {rel(flux.sum(), mf.sum()):.2e} "
+          f"state_flux {rel(psi.astype(np.float64).sum(), mp.sum()):.2e}", flush=True)
+
+
+def test_config4_batched_full(gpu_ready):
+    prob = smoc.generate(0, G=100, R=6750, F=5, seed=42, lognormal_tracks=8192)
+    print(f"case configs[3]: 8192 lognormal tracks, {prob.S} segments, autotuned graph",
+          flush=True)
+    with smoc.Context(100, 6750, 5) as ctx:
+        ctx.load(prob)
+        tm = ctx.run(autotune=1, use_graph=1)
+        flux, psi = ctx.results()
+        u = tm.used
+        print(f"  tuned chunk {u.chunk} persistent {u.persistent} gpl {u.groups_per_lane} "
+              f"wpb {u.waves_per_block}: {tm.total_ms:.2f} ms", flush=True)
+    of, op = oracle(prob)
+    assert_close(flux, psi, of, op)
+
+
+def test_config5_large_properties(gpu_ready):
+    R, G = 13_500, 128
+    prob = smoc.generate(200_000_000, G=G, R=R, F=5, seed=42, sorted_stream=True)
+    print(f"case configs[4]: {prob.S} x {G} sorted, bf16, det FP64", flush=True)
+    with smoc.Context(G, R, 5, storage="bf16", tally="det-f64") as ctx:
+        ctx.load(prob)
+        ctx.run()
+        f0, p0 = ctx.results()
+        ctx.run(groups_per_lane=1, waves_per_block=8)
+        f1, p1 = ctx.results()
+    assert np.array_equal(f0, f1) and np.array_equal(p0, p1)  # bit-exact run to run
+    with smoc.Context(G, R, 5, storage="bf16", tally="atomic-f64") as ctx:
+        ctx.load(prob)
+        ctx.run()
+        fa, pa = ctx.results()
+    assert np.array_equal(pa, p0)
+    assert normwise(fa, f0) < 1e-12  # two tally paths, FP64 accumulation
+    # psi_out of sampled tracks against the FP64 oracle on those tracks
+    rng = np.random.default_rng(0)
+    picks = np.sort(rng.choice(prob.T, 64, replace=False))
+    q = quantized(prob, "bf16")
+    segs = np.concatenate([np.arange(prob.track_off[t], prob.track_off[t + 1]) for t in picks])
+    lens = np.diff(prob.track_off)[picks]
+    sub = smoc.Problem(G, R, 5, q.sigT, q.fine_source, prob.len[segs], prob.region[segs],
+                       prob.z[segs], np.concatenate([[0], np.cumsum(lens)]).astype(np.int64),
+                       prob.psi_in[picks])
+    _, op = pyoracle.sweep_f64(sub, pyoracle.EXP_NATIVE, nthreads=THREADS)
+    print(f"  sampled psi normwise {normwise(p0[picks], op):.2e}", flush=True)
+    assert normwise(p0[picks], op) < 1e-5
