@@ -74,21 +74,22 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // Chunking of long tracks for one chunk length C.
 struct Plan {
     int chunk = 0;
-    int64_t n_items = 0, n_xfer = 0;
+    int64_t n_items = 0, n1 = 0, n2 = 0;
     DevBuf<int64_t> items;           // [n_items][4]
-    DevBuf<int64_t> xfer_list;       // items needing a transfer pass
+    DevBuf<int64_t> list1, list2;    // phase-1 / phase-2 work entries
     DevBuf<int64_t> track_item_off;  // [T+1]
     DevBuf<float> chunk_psi;         // [n_items][Gp]
     DevBuf<float> xfer;              // [n_items][2][Gp]
     void release()
     {
         items.release();
-        xfer_list.release();
+        list1.release();
+        list2.release();
         track_item_off.release();
         chunk_psi.release();
         xfer.release();
         chunk = 0;
-        n_items = n_xfer = 0;
+        n_items = n1 = n2 = 0;
     }
 };
 
@@ -579,12 +580,15 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
 }
 
 // Work items of a chunked sweep: every track becomes ceil(len / C) chunks
-// (an empty track one empty chunk); chunks that are not their track's last
-// need a transfer pass.
+// (an empty track one empty chunk).  Phase 1 sweeps each track's first chunk
+// (a multi-chunk track's first chunk also emits its psi_out as a transfer)
+// and runs the transfer pass of the middle chunks; phase 2 sweeps the
+// remaining chunks.  Both lists are ordered longest first.
 static int build_plan(smoc_ctx *c, int C)
 {
     Plan &pl = c->plan;
-    std::vector<int64_t> items, xl, tio((size_t)c->T + 1);
+    std::vector<int64_t> items, tio((size_t)c->T + 1);
+    std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
     const std::vector<int64_t> &off = c->h_track_off;
     int64_t n = 0;
     for (int64_t t = 0; t < c->T; t++) {
@@ -597,32 +601,52 @@ static int build_plan(smoc_ctx *c, int C)
             items.push_back(cb);
             items.push_back(ce);
             items.push_back(cb);  // first run id: runs are segments (checked by caller)
-            if (k + 1 < nc)
-                xl.push_back(n);
+            if (k == 0)
+                p1.emplace_back(ce - cb, (n << 2) | (nc > 1 ? 2 : 0));
+            else
+                p2.emplace_back(ce - cb, n << 2);
+            if (k > 0 && k + 1 < nc)
+                p1.emplace_back(ce - cb, (n << 2) | 1);
         }
     }
     tio[c->T] = n;
-    int rc = pl.items.alloc(items.size());
+    auto longest_first = [](const std::pair<int64_t, int64_t> &x,
+                            const std::pair<int64_t, int64_t> &y) { return x.first > y.first; };
+    std::stable_sort(p1.begin(), p1.end(), longest_first);
+    std::stable_sort(p2.begin(), p2.end(), longest_first);
+    std::vector<int64_t> l1, l2;
+    for (auto &x : p1)
+        l1.push_back(x.second);
+    for (auto &x : p2)
+        l2.push_back(x.second);
+    int rc = pl.items.alloc(std::max<size_t>(items.size(), 4));
     if (rc == SMOC_OK)
-        rc = pl.xfer_list.alloc(std::max<size_t>(xl.size(), 1));
+        rc = pl.list1.alloc(std::max<size_t>(l1.size(), 1));
+    if (rc == SMOC_OK)
+        rc = pl.list2.alloc(std::max<size_t>(l2.size(), 1));
     if (rc == SMOC_OK)
         rc = pl.track_item_off.alloc(tio.size());
     if (rc == SMOC_OK)
-        rc = pl.chunk_psi.alloc((size_t)n * c->Gp);
+        rc = pl.chunk_psi.alloc((size_t)std::max<int64_t>(n, 1) * c->Gp);
     if (rc == SMOC_OK)
         rc = pl.xfer.alloc((size_t)std::max<int64_t>(n, 1) * 2 * c->Gp);
     if (rc != SMOC_OK)
         return rc;
-    HIPCHK(hipMemcpy(pl.items.p, items.data(), sizeof(int64_t) * items.size(),
-                     hipMemcpyHostToDevice));
-    if (!xl.empty())
-        HIPCHK(hipMemcpy(pl.xfer_list.p, xl.data(), sizeof(int64_t) * xl.size(),
+    if (!items.empty())
+        HIPCHK(hipMemcpy(pl.items.p, items.data(), sizeof(int64_t) * items.size(),
+                         hipMemcpyHostToDevice));
+    if (!l1.empty())
+        HIPCHK(hipMemcpy(pl.list1.p, l1.data(), sizeof(int64_t) * l1.size(),
+                         hipMemcpyHostToDevice));
+    if (!l2.empty())
+        HIPCHK(hipMemcpy(pl.list2.p, l2.data(), sizeof(int64_t) * l2.size(),
                          hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(pl.track_item_off.p, tio.data(), sizeof(int64_t) * tio.size(),
                      hipMemcpyHostToDevice));
     pl.chunk = C;
     pl.n_items = n;
-    pl.n_xfer = (int64_t)xl.size();
+    pl.n1 = (int64_t)l1.size();
+    pl.n2 = (int64_t)l2.size();
     return SMOC_OK;
 }
 
@@ -688,38 +712,45 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     if (ev)
         HIPCHK(hipEventRecord(ev[1], c->stream));
     if (chunked) {
+        // phase 1: first chunks (swept from psi_in) + transfers of middle chunks
         Plan &pl = c->plan;
         a.items = pl.items.p;
-        a.chunk_psi = pl.chunk_psi.p;
         a.xfer = pl.xfer.p;
-        if (pl.n_xfer > 0) {
-            smoc_sweep_args x = a;
-            x.xfer_mode = 1;
-            x.item_list = pl.xfer_list.p;
-            x.chunk_psi = nullptr;
-            x.n_items = pl.n_xfer;
-            if (l.persistent) {
-                x.queue = c->queue.p;
-                LAUNCHCHK(smoc_launch_zero_i64(c->queue.p, 1, c->stream));
-            }
-            LAUNCHCHK(smoc_launch_sweep(&x, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
-                                        (int)grid_for(pl.n_xfer), c->stream));
+        smoc_sweep_args p1 = a;
+        p1.xfer_mode = 1;
+        p1.item_list = pl.list1.p;
+        p1.chunk_psi = nullptr;
+        p1.n_items = pl.n1;
+        if (l.persistent) {
+            p1.queue = c->queue.p;
+            LAUNCHCHK(smoc_launch_zero_i64(c->queue.p, 1, c->stream));
         }
-        smoc_scan_args sa{pl.track_item_off.p, pl.xfer.p, c->psi_in.p, pl.chunk_psi.p, c->T,
-                          c->Gp, slices};
-        LAUNCHCHK(smoc_launch_scan(&sa, gpl, c->stream));
-        a.n_items = pl.n_items;
+        LAUNCHCHK(smoc_launch_sweep(&p1, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
+                                    (int)grid_for(pl.n1), c->stream));
+        if (pl.n2 > 0) {
+            smoc_scan_args sa{pl.track_item_off.p, pl.xfer.p, c->psi_in.p, pl.chunk_psi.p, c->T,
+                              c->Gp, slices};
+            LAUNCHCHK(smoc_launch_scan(&sa, gpl, c->stream));
+        }
+        if (ev)
+            HIPCHK(hipEventRecord(ev[2], c->stream));
+        // phase 2: the remaining chunks from the scanned psi
+        a.item_list = pl.list2.p;
+        a.chunk_psi = pl.chunk_psi.p;
+        a.n_items = pl.n2;
     } else {
         a.n_items = c->T;
+        if (ev)
+            HIPCHK(hipEventRecord(ev[2], c->stream));
     }
-    if (ev)
-        HIPCHK(hipEventRecord(ev[2], c->stream));
-    if (l.persistent) {
-        a.queue = c->queue.p + 1;
-        LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
+    if (a.n_items > 0) {
+        if (l.persistent) {
+            a.queue = c->queue.p + 1;
+            LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
+        }
+        LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
+                                    (int)grid_for(a.n_items), c->stream));
     }
-    LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
-                                (int)grid_for(a.n_items), c->stream));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], c->stream));
     if (c->det()) {

@@ -39,11 +39,13 @@ typedef struct {
     /* chunked sweeps (tracks split into chunks) */
     const int64_t *items;        /* NULL: item i = track i; else [n][4]: track, begin,
                                     end, first run id */
-    const int64_t *item_list;    /* NULL: work entry w = item w; else item ids */
+    const int64_t *item_list;    /* NULL: work entry w = item w; else entries
+                                    item id << 2 | flags (1: transfer pass,
+                                    2: sweep and emit psi_out as a transfer) */
     const float *chunk_psi;      /* [items][Gp] incoming psi of each chunk, or NULL */
     float *xfer;                 /* transfer pass output [items][2][Gp]: B, M */
     int64_t n_items;             /* work entries: tracks, chunks or list length */
-    int xfer_mode;               /* 1: transfer pass (no tally) */
+    int xfer_mode;               /* 1: phase-1 launch, mode chosen per entry */
     int64_t *queue;              /* persistent mode: work counter, or NULL    */
     float tab_dx, tab_max;
     int tab_n;

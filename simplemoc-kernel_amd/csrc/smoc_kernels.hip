@@ -325,7 +325,13 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // affine in psi), computed from psi_in = 0 and a running product of 1 - ev;
 // no tally.  A scan over a track's chunks turns the transfers into every
 // chunk's incoming psi.
-enum { MODE_SWEEP = 0, MODE_XFER = 1 };
+//
+// Chunked runs go in two launches.  Phase 1 sweeps every track's first chunk
+// (its incoming psi is psi_in) and, in the same launch, runs the transfer pass
+// of the middle chunks; phase 2, after the scan, sweeps the remaining chunks.
+// MODE_MIXED kernels pick the mode per work entry from its ITEM_XFER flag.
+enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2 };
+enum { ITEM_XFER = 1, ITEM_EMIT = 2 };
 
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
@@ -340,7 +346,10 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
     const int F = a.F;
     const int64_t w = a.slices == 1 ? item : item / a.slices;
     const int sl = (int)(item - w * a.slices);
-    const int64_t it = a.item_list ? uniform64(a.item_list[w]) : w;
+    // list entries: item id << 2 | ITEM_* flags
+    const int64_t entry = a.item_list ? uniform64(a.item_list[w]) : (w << 2);
+    const int64_t it = entry >> 2;
+    const bool emit_xfer = (entry & ITEM_EMIT) != 0;
 
     int64_t t, kb, ke, run = 0;
     if (items) {
@@ -445,12 +454,14 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
             s1 = s3;
         }
     }
-    if constexpr (MODE == MODE_XFER) {
+    if (MODE == MODE_XFER || emit_xfer) {
+        // transfer of this chunk; a swept first chunk knows its psi_out
+        // exactly and passes it on as B with M = 0
         float *xb = a.xfer + (2 * it) * Gp, *xm = a.xfer + (2 * it + 1) * Gp;
 #pragma unroll
         for (int j = 0; j < GPL; j++) {
             xb[g[j]] = psi[j];
-            xm[g[j]] = mprod[j];
+            xm[g[j]] = MODE == MODE_XFER ? mprod[j] : 0.f;
         }
     } else if (!items || ke == uniform64(track_off[t + 1])) {
         // chunks other than a track's last carry no psi_out
@@ -458,6 +469,29 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 #pragma unroll
         for (int j = 0; j < GPL; j++)
             pout[g[j]] = psi[j];
+    }
+}
+
+template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
+__device__ __forceinline__ void dispatch_item(const smoc_sweep_args &a,
+                                              const float *__restrict__ len,
+                                              const int32_t *__restrict__ region,
+                                              const float *__restrict__ z,
+                                              const int64_t *__restrict__ track_off,
+                                              const int64_t *__restrict__ items, int64_t item,
+                                              int lane, const ExpTab &tb)
+{
+    if constexpr (MODE == MODE_MIXED) {
+        const int64_t w = a.slices == 1 ? item : item / a.slices;
+        if (uniform64(a.item_list[w]) & ITEM_XFER)
+            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_XFER>(a, len, region, z, track_off,
+                                                                 items, item, lane, tb);
+        else
+            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, len, region, z, track_off,
+                                                                  items, item, lane, tb);
+    } else {
+        sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
+                                                        item, lane, tb);
     }
 }
 
@@ -493,8 +527,8 @@ __global__ void __launch_bounds__(1024)
     const int64_t item = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * wpb + wave;
     if (item >= a.n_items * a.slices)
         return;
-    sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items, item,
-                                                    lane, tb);
+    dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items, item,
+                                                       lane, tb);
 }
 
 // Persistent variant: each wave pulls work items from a device counter until
@@ -516,8 +550,8 @@ __global__ void __launch_bounds__(1024)
         const int64_t item = uniform64((int64_t)__shfl(got, 0));
         if (item >= n_work)
             break;
-        sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
-                                                        item, lane, tb);
+        dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
+                                                           item, lane, tb);
     }
 }
 
@@ -711,6 +745,8 @@ int launch_sweep_m(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS>
 int launch_sweep_r(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 {
+    if (a->xfer_mode)
+        return launch_sweep_m<STORE, GPL, EXPM, TALLY, RUNS, MODE_MIXED>(a, wpb, grid, s);
     return launch_sweep_m<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, wpb, grid, s);
 }
 
@@ -725,9 +761,6 @@ int launch_sweep_t(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 template <int STORE, int GPL, int EXPM>
 int launch_sweep_tally(const smoc_sweep_args *a, int tally, int wpb, int grid, hipStream_t s)
 {
-    if (a->xfer_mode)  // no tally in the transfer pass
-        return launch_sweep_m<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F32, false, MODE_XFER>(a, wpb,
-                                                                                    grid, s);
     switch (tally) {
     case SMOC_TALLY_ATOMIC_F32:
         return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_ATOMIC_F32>(a, wpb, grid, s);
