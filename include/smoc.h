@@ -44,6 +44,7 @@ extern "C" {
 #define SMOC_ERR_HIP (-3)         /* a HIP runtime call failed */
 #define SMOC_ERR_STATE (-4)       /* call out of order (e.g. run before set_*) */
 #define SMOC_ERR_NOMEM (-5)       /* device or host allocation failed */
+#define SMOC_ERR_RANGE (-6)       /* a FIXED64 tally left its fixed-point range */
 
 /* ---- variants ---------------------------------------------------------- */
 /* Storage of sigT and fine_source in HBM; arithmetic is FP32 in every case. */
@@ -71,11 +72,18 @@ extern "C" {
  *               in one (region, interval) is summed in FP64 registers and
  *               stored; a segmented reduction then sums the runs of every
  *               (region, interval) in a fixed order.  Bit-exact run to run.
- *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums. */
+ *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums.
+ *   FIXED64     deterministic: each run is rounded to a 64-bit fixed-point
+ *               integer (2^-32 resolution, |fine_flux| < 2^31) and added with
+ *               integer atomics, which are associative, so the result is
+ *               bit-exact run to run in any order; converted to FP64 after
+ *               the sweep.  smoc_run fails with SMOC_ERR_RANGE if a value
+ *               left the range. */
 #define SMOC_TALLY_ATOMIC_F32 0
 #define SMOC_TALLY_ATOMIC_F64 1
 #define SMOC_TALLY_DET_F64 2
 #define SMOC_TALLY_DET_KAHAN 3
+#define SMOC_TALLY_FIXED64 4
 
 typedef struct smoc_ctx smoc_ctx;
 

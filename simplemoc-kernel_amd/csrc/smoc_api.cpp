@@ -119,6 +119,7 @@ struct smoc_ctx {
     DevBuf<float> psi_in, psi_out;  // [T][Gp]
     bool segments_set = false;
     bool runs = false;  // sorted-enough stream: sum runs in registers
+    float fix_limit = 0.f;
 
     // deterministic tally bookkeeping
     int64_t n_runs = 0;
@@ -138,8 +139,13 @@ struct smoc_ctx {
     int64_t n_fsr() const { return (int64_t)p.regions * p.fine_axial; }
     bool flux_f64() const
     {
-        return p.tally == SMOC_TALLY_ATOMIC_F64 || p.tally == SMOC_TALLY_DET_F64;
+        return p.tally == SMOC_TALLY_ATOMIC_F64 || p.tally == SMOC_TALLY_DET_F64 ||
+               p.tally == SMOC_TALLY_FIXED64;
     }
+    bool fixed() const { return p.tally == SMOC_TALLY_FIXED64; }
+    // fine_flux starts 256 bytes into its buffer; with FIXED64 the 8 bytes
+    // just before it count saturated tallies
+    int64_t *fixed_flux() const { return reinterpret_cast<int64_t *>(flux.p + 256); }
     bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
 };
 
@@ -244,7 +250,7 @@ int smoc_create(smoc_ctx **out, const smoc_params *p)
     }
     if (p->storage < SMOC_STORE_F32 || p->storage > SMOC_STORE_F16 ||
         p->exp_mode < SMOC_EXP_NATIVE || p->exp_mode > SMOC_EXP_ACCURATE ||
-        p->tally < SMOC_TALLY_ATOMIC_F32 || p->tally > SMOC_TALLY_DET_KAHAN) {
+        p->tally < SMOC_TALLY_ATOMIC_F32 || p->tally > SMOC_TALLY_FIXED64) {
         smoc_set_error("smoc_create: unknown storage %d / exp_mode %d / tally %d", p->storage,
                        p->exp_mode, p->tally);
         return SMOC_ERR_ARG;
@@ -371,7 +377,7 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
     const int64_t nrow = c->n_fsr();
     int rc = c->tuples.alloc((size_t)nrow * c->Gp * smoc_tuple_bytes(c->p.storage));
     if (rc == SMOC_OK)
-        rc = c->flux.alloc((size_t)nrow * c->Gp * (c->flux_f64() ? 8 : 4));
+        rc = c->flux.alloc((size_t)nrow * c->Gp * (c->flux_f64() ? 8 : 4) + 256);
     if (rc != SMOC_OK)
         return rc;
     // FP32 masters are staged on the device, packed (and quantised) there
@@ -512,6 +518,20 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
     // A stream sorted by region keeps most neighbours in the same region;
     // then runs are summed in registers.
     c->runs = S > 1 && 2 * same >= S;
+    // FIXED64: no fine_flux sum may reach 2^31, so bound each contribution by
+    // 2^31 / (most segments landing in one region interval)
+    c->fix_limit = 0.f;
+    if (c->fixed()) {
+        std::vector<int32_t> cnt((size_t)c->n_fsr(), 0);
+        int32_t most = 1;
+        for (int64_t k = 0; k < S; k++) {
+            int f;
+            float zin;
+            smoc_decode_z(z[k], c->p.fine_axial, &f, &zin);
+            most = std::max(most, ++cnt[(size_t)region[k] * c->p.fine_axial + f]);
+        }
+        c->fix_limit = (float)(2147483647.0 / most * 0.999);
+    }
 
     int rc = c->len.alloc((size_t)S);
     if (rc == SMOC_OK)
@@ -683,7 +703,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     a.track_off = c->track_off.p;
     a.psi_in = c->psi_in.p;
     a.psi_out = c->psi_out.p;
-    a.flux = c->det() ? (void *)c->part.p : (void *)c->flux.p;
+    a.flux = c->det() ? (void *)c->part.p : (void *)(c->flux.p + 256);
     a.track_run_off = c->track_run_off.p;
     a.table = c->table.p;
     a.tab_dx = c->tab_dx;
@@ -694,6 +714,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     a.F = c->p.fine_axial;
     a.slices = slices;
     a.runs = c->runs ? 1 : 0;
+    a.fix_limit = c->fix_limit;
     // persistent grids: as many workgroups as can be resident (8 waves per
     // SIMD, 4 SIMDs per CU)
     const int64_t resident_blocks = (int64_t)c->cu_count * std::max(1, 32 / wpb);
@@ -706,8 +727,11 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
 
     if (ev)
         HIPCHK(hipEventRecord(ev[0], c->stream));
-    if (!c->det())
-        LAUNCHCHK(smoc_launch_reset(c->flux.p, c->flux_init.p, nrow * c->Gp,
+    if (c->fixed())  // counter (first 8 bytes) and the fixed-point sums
+        LAUNCHCHK(smoc_launch_zero_i64(reinterpret_cast<int64_t *>(c->flux.p),
+                                       nrow * c->Gp + 32, c->stream));
+    else if (!c->det())
+        LAUNCHCHK(smoc_launch_reset(c->flux.p + 256, c->flux_init.p, nrow * c->Gp,
                                     c->flux_f64() ? 1 : 0, c->stream));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], c->stream));
@@ -759,13 +783,16 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
         r.fsr_run_off = c->fsr_run_off.p;
         r.fsr_run_idx = c->fsr_run_idx.p;
         r.init = c->flux_init.p;
-        r.flux = c->flux.p;
+        r.flux = c->flux.p + 256;
         r.n_fsr = nrow;
         r.Gp = c->Gp;
         r.slices = slices;
         r.single = c->runs ? 0 : 1;
         LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, 4, c->stream));
     }
+    if (c->fixed())
+        LAUNCHCHK(smoc_launch_fixed_to_f64(c->fixed_flux(), c->flux_init.p, nrow * c->Gp,
+                                           c->stream));
     if (ev)
         HIPCHK(hipEventRecord(ev[4], c->stream));
     return SMOC_OK;
@@ -957,6 +984,15 @@ int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
     int rc = run_once(c, l, &tm);
     if (rc != SMOC_OK)
         return rc;
+    if (c->fixed()) {
+        uint32_t overflows = 0;
+        HIPCHK(hipMemcpy(&overflows, c->flux.p + 248, sizeof(overflows), hipMemcpyDeviceToHost));
+        if (overflows) {
+            smoc_set_error("smoc_run: %u FIXED64 tallies left the range |x| < 2^31 and saturated",
+                           overflows);
+            return SMOC_ERR_RANGE;
+        }
+    }
     c->ran = true;
     if (timing) {
         const int G = c->p.egroups;
@@ -984,11 +1020,11 @@ int smoc_get_results(smoc_ctx *c, double *fine_flux, float *psi_out)
     const int64_t nrow = c->n_fsr();
     if (fine_flux) {
         if (c->flux_f64()) {
-            HIPCHK(hipMemcpy2D(fine_flux, sizeof(double) * G, c->flux.p, sizeof(double) * Gp,
+            HIPCHK(hipMemcpy2D(fine_flux, sizeof(double) * G, c->flux.p + 256, sizeof(double) * Gp,
                                sizeof(double) * G, (size_t)nrow, hipMemcpyDeviceToHost));
         } else {
             std::vector<float> tmp((size_t)nrow * G);
-            HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * G, c->flux.p, sizeof(float) * Gp,
+            HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * G, c->flux.p + 256, sizeof(float) * Gp,
                                sizeof(float) * G, (size_t)nrow, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < tmp.size(); i++)
                 fine_flux[i] = tmp[i];

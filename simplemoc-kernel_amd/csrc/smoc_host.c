@@ -41,6 +41,7 @@ const char *smoc_strerror(int code)
     case SMOC_ERR_HIP: return "HIP runtime error";
     case SMOC_ERR_STATE: return "call out of order";
     case SMOC_ERR_NOMEM: return "out of memory";
+    case SMOC_ERR_RANGE: return "fixed-point tally out of range";
     default: return "unknown error";
     }
 }

@@ -33,7 +33,9 @@ typedef struct {
     const int64_t *track_off;    /* [T+1] */
     const float *psi_in;         /* [T][Gp] */
     float *psi_out;              /* [T][Gp] */
-    void *flux;                  /* ATOMIC_*: fine_flux [R*F][Gp]; DET_*: run partials [runs][Gp] */
+    void *flux;                  /* ATOMIC_*: fine_flux [R*F][Gp]; DET_*: run partials
+                                    [runs][Gp]; FIXED64: int64 fine_flux whose
+                                    preceding 8 bytes count overflows */
     const int64_t *track_run_off;/* DET_*: first run id of each track [T+1]  */
     const float *table;          /* exp table, 2N floats (EXP_TABLE)          */
     /* chunked sweeps (tracks split into chunks) */
@@ -52,6 +54,7 @@ typedef struct {
     int G, Gp, F, slices;        /* Gp: padded row stride (multiple of 128) */
     int runs;                    /* 1: sum runs of equal (region, interval) in
                                     registers; 0: every segment is a run */
+    float fix_limit;             /* FIXED64: bound on one segment's tally */
 } smoc_sweep_args;
 
 typedef struct {
@@ -90,6 +93,7 @@ int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_
                        void *stream);
 int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream);
 int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
+int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);
 /* bytes per packed tuple for a storage type */
 int smoc_tuple_bytes(int storage);

@@ -168,12 +168,12 @@ struct EvTerms {
 
 // Taylor series for 0 <= tau < 1: r4 = sum_{k>=0} (-tau)^k / (k+4)!,
 // r = 1/2 - tau/6 + tau^2 r4; then ev = tau - tau^2 r, tme = tau^2 r and
-// poly = 6 tau^4 r4 without subtracting nearly equal numbers.  Degree 7 in
-// r4 truncates at tau^8/12! * 4! < 5e-8 relative for tau < 1.
+// poly = 6 tau^4 r4 without subtracting nearly equal numbers.  Degree 6 in
+// r4 truncates at tau^7/11! * 4! < 7e-7 relative to r4 for tau < 1, which
+// moves ev by under 3e-8 absolute.
 __device__ __forceinline__ EvTerms ev_series(float tau)
 {
-    float r4 = -1.f / 39916800.f;
-    r4 = r4 * tau + 1.f / 3628800.f;
+    float r4 = 1.f / 3628800.f;
     r4 = r4 * tau - 1.f / 362880.f;
     r4 = r4 * tau + 1.f / 40320.f;
     r4 = r4 * tau - 1.f / 5040.f;
@@ -224,8 +224,13 @@ __device__ __forceinline__ EvTerms ev_terms(float tau, const ExpTab &tb)
     }
 }
 
-// Attenuation of one (segment, group): kernel.c:200-332.  Returns the tally
-// (weight * flux_integral) and updates psi.
+// Attenuation of one (segment, group): kernel.c:200-332.  Updates psi and
+// returns the flux integral (the caller applies the 0.5 weight, an exact
+// scaling, once per flushed run).  The first term of the integral,
+// (q0 tau + (sigT psi - q0) ev) / sigT^2, is evaluated as
+// q0 (tau - ev) / sigT^2 + psi ev / sigT, which shares ev / sigT and
+// (tau - ev) / sigT^2 with the psi update and uses the cancellation-free
+// tau - ev of ev_terms.
 template <int EXPM>
 __device__ __forceinline__ float attenuate(float sig, float q0, float q1, float q2, float ds,
                                            float &psi, float &one_m_ev, const ExpTab &tb)
@@ -235,13 +240,15 @@ __device__ __forceinline__ float attenuate(float sig, float q0, float q1, float 
     const float ev = x.ev;
     const float i1 = rcp(sig);
     const float i2 = i1 * i1;
-    const float i3 = i2 * i1;
-    const float reuse = tau * (tau - 2.f) + 2.f * ev * i3;
-    const float flux_integral = (q0 * tau + (sig * psi - q0) * ev) * i2 + q1 * SMOC_MU * reuse +
-                                q2 * SMOC_MU2 * x.poly * (i2 * i2 * (1.f / 3.f));
-    psi = q0 * ev * i1 + q1 * SMOC_MU * x.tme * i2 + q2 * SMOC_MU2 * reuse + psi * x.one_m_ev;
+    const float e1 = ev * i1;     // ev / sigT
+    const float m2 = x.tme * i2;  // (tau - ev) / sigT^2
+    const float reuse = tau * (tau - 2.f) + 2.f * e1 * i2;
+    const float q1m = q1 * SMOC_MU, q2m = q2 * SMOC_MU2;
+    const float flux_integral =
+        q0 * m2 + psi * e1 + q1m * reuse + q2m * x.poly * (i2 * i2 * (1.f / 3.f));
+    psi = q0 * e1 + q1m * m2 + q2m * reuse + psi * x.one_m_ev;
     one_m_ev = x.one_m_ev;
-    return SMOC_WEIGHT * flux_integral;
+    return flux_integral;
 }
 
 // ---- tally accumulators ------------------------------------------------------
@@ -253,7 +260,7 @@ template <bool SINGLE> struct Acc<SMOC_TALLY_ATOMIC_F32, SINGLE> {
     __device__ __forceinline__ void add(float x) { s += x; }
     __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
     {
-        unsafeAtomicAdd(static_cast<float *>(flux) + row * G + g, s);
+        unsafeAtomicAdd(static_cast<float *>(flux) + row * G + g, SMOC_WEIGHT * s);
         s = 0.f;
     }
 };
@@ -262,7 +269,7 @@ template <bool SINGLE> struct Acc<SMOC_TALLY_ATOMIC_F64, SINGLE> {
     __device__ __forceinline__ void add(float x) { s += (double)x; }
     __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
     {
-        unsafeAtomicAdd(static_cast<double *>(flux) + row * G + g, s);
+        unsafeAtomicAdd(static_cast<double *>(flux) + row * G + g, (double)SMOC_WEIGHT * s);
         s = 0.0;
     }
 };
@@ -271,7 +278,7 @@ template <> struct Acc<SMOC_TALLY_DET_F64, false> {
     __device__ __forceinline__ void add(float x) { s += (double)x; }
     __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
     {
-        static_cast<double *>(part)[run * G + g] = s;
+        static_cast<double *>(part)[run * G + g] = (double)SMOC_WEIGHT * s;
         s = 0.0;
     }
 };
@@ -287,17 +294,52 @@ template <> struct Acc<SMOC_TALLY_DET_KAHAN, false> {
     }
     __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
     {
-        static_cast<float2 *>(part)[run * G + g] = make_float2(s, c);
+        static_cast<float2 *>(part)[run * G + g] = make_float2(SMOC_WEIGHT * s, SMOC_WEIGHT * c);
         s = 0.f;
         c = 0.f;
     }
 };
+// 64-bit fixed point with 32 fractional bits: integer atomics are
+// associative, so the sum is bit-exact whatever the order.  The host bounds
+// every contribution by lim = (2^31 - 1) / (most segments in one region
+// interval), so no sum can wrap; a contribution beyond lim (or NaN) raises
+// the overflow counter and the run reports SMOC_ERR_RANGE.
+constexpr double kFixScale = 4294967296.0;  // 2^32
+template <bool SINGLE> struct Acc<SMOC_TALLY_FIXED64, SINGLE> {
+    double s = 0.0;
+    bool bad = false;
+    float lim = 0.f;
+    __device__ __forceinline__ void add(float x)
+    {
+        bad |= !(fabsf(x) <= lim);  // x is unweighted: lim is twice the bound
+        s += (double)x;
+    }
+    __device__ __forceinline__ void flush(void *flux, int64_t row, int64_t, int G, int g)
+    {
+        if (bad) {
+            atomicAdd(reinterpret_cast<unsigned int *>(static_cast<int64_t *>(flux) - 1), 1u);
+            bad = false;
+            s = 0.0;
+        }
+        const long long q = __double2ll_rn((double)SMOC_WEIGHT * s * kFixScale);
+        atomicAdd(reinterpret_cast<unsigned long long *>(static_cast<int64_t *>(flux) + row * G + g),
+                  (unsigned long long)q);
+        s = 0.0;
+    }
+};
+template <typename A> __device__ __forceinline__ void acc_init(A &, float) {}
+template <bool SINGLE>
+__device__ __forceinline__ void acc_init(Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, float lim)
+{
+    acc.lim = lim / SMOC_WEIGHT;
+}
+
 template <int TALLY> struct AccSingle {
     float s = 0.f;
     __device__ __forceinline__ void add(float x) { s = x; }
     __device__ __forceinline__ void flush(void *part, int64_t, int64_t run, int G, int g)
     {
-        static_cast<float *>(part)[run * G + g] = s;
+        static_cast<float *>(part)[run * G + g] = SMOC_WEIGHT * s;
     }
 };
 template <> struct Acc<SMOC_TALLY_DET_F64, true> : AccSingle<SMOC_TALLY_DET_F64> {};
@@ -379,6 +421,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         gi[j] = g[j] < G ? g[j] : G - 1;
         psi[j] = MODE == MODE_XFER ? 0.f : pin[gi[j]];
         mprod[j] = 1.f;
+        acc_init(acc[j], a.fix_limit);
     }
 
     const raw_t *__restrict__ tup = static_cast<const raw_t *>(a.tuples);
@@ -391,11 +434,13 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
     // compiler's memory-counter waits stay exact: a load issued on only one
     // side of a branch makes every later wait assume the shorter path and
     // stall on the loads just issued.  Past the end, row 0 is read and unused.
+    // uniform row base + 32-bit lane offset: global_load with an SGPR base
     auto gather = [&](raw_t (&buf)[GPL], const Seg &sg) {
         const int64_t row = sg.row >= 0 ? sg.row : 0;
+        const raw_t *__restrict__ base = tup + row * Gp;
 #pragma unroll
         for (int j = 0; j < GPL; j++)
-            buf[j] = tup[row * Gp + gi[j]];
+            buf[j] = base[(uint32_t)gi[j]];
     };
     auto raw_at = [&](int64_t k) {
         return k < ke ? load_raw(len, region, z, k) : RawSeg{0.f, 0.f, -1};
@@ -653,6 +698,16 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
             dst[base + u * 256] = v[u];
 }
 
+// FIXED64 result, in place: int64 fixed point -> init + value as FP64.
+__global__ void __launch_bounds__(256) k_fixed_to_f64(int64_t *flux, const float *init, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double v = (double)flux[i] * (1.0 / kFixScale);
+        reinterpret_cast<double *>(flux)[i] = init ? (double)init[i] + v : v;
+    }
+}
+
 __global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -770,6 +825,8 @@ int launch_sweep_tally(const smoc_sweep_args *a, int tally, int wpb, int grid, h
         return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_DET_F64>(a, wpb, grid, s);
     case SMOC_TALLY_DET_KAHAN:
         return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_DET_KAHAN>(a, wpb, grid, s);
+    case SMOC_TALLY_FIXED64:
+        return launch_sweep_t<STORE, GPL, EXPM, SMOC_TALLY_FIXED64>(a, wpb, grid, s);
     }
     return (int)hipErrorInvalidValue;
 }
@@ -842,6 +899,16 @@ extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int gri
         return 0;
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n,
+                                        void *stream)
+{
+    if (n == 0)
+        return 0;
+    hipLaunchKernelGGL(k_fixed_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), flux, init, n);
     return (int)hipGetLastError();
 }
 

@@ -68,7 +68,7 @@ static void print_CLI_error(void)
     printf("  -d <device ID>        HIP device ID number\n");
     printf("  --storage f32|bf16|f16             sigT / fine_source storage\n");
     printf("  --exp native|table|accurate        1-exp(-tau) evaluation\n");
-    printf("  --tally atomic-f32|atomic-f64|det-f64|det-kahan\n");
+    printf("  --tally atomic-f32|atomic-f64|det-f64|det-kahan|fixed64\n");
     printf("  --track-len L          fixed track length in segments (default 100)\n");
     printf("  --lognormal T          T tracks of lognormal(4000, 1.0) length in [16, 200000]\n");
     printf("  --sorted               sort the stream by (region, fine interval)\n");
@@ -94,7 +94,8 @@ static void read_CLI(int argc, char *argv[], options *o)
 {
     static const char *const stores[] = {"f32", "bf16", "f16"};
     static const char *const exps[] = {"native", "table", "accurate"};
-    static const char *const tallies[] = {"atomic-f32", "atomic-f64", "det-f64", "det-kahan"};
+    static const char *const tallies[] = {"atomic-f32", "atomic-f64", "det-f64", "det-kahan",
+                                          "fixed64"};
     for (int i = 1; i < argc; i++) {
         const char *arg = argv[i];
 #define NEXT() (++i < argc ? argv[i] : (print_CLI_error(), ""))
@@ -115,7 +116,7 @@ static void read_CLI(int argc, char *argv[], options *o)
         else if (strcmp(arg, "--exp") == 0)
             o->exp_mode = parse_choice(NEXT(), exps, 3);
         else if (strcmp(arg, "--tally") == 0)
-            o->tally = parse_choice(NEXT(), tallies, 4);
+            o->tally = parse_choice(NEXT(), tallies, 5);
         else if (strcmp(arg, "--track-len") == 0)
             o->track_len = atoi(NEXT());
         else if (strcmp(arg, "--lognormal") == 0)
@@ -229,7 +230,7 @@ int main(int argc, char *argv[])
     printf("%-25s%s\n", "Storage:",
            o.storage == SMOC_STORE_F32 ? "f32" : (o.storage == SMOC_STORE_BF16 ? "bf16" : "f16"));
     printf("%-25s%s\n", "Tally:",
-           (const char *[]){"atomic-f32", "atomic-f64", "det-f64", "det-kahan"}[o.tally]);
+           (const char *[]){"atomic-f32", "atomic-f64", "det-f64", "det-kahan", "fixed64"}[o.tally]);
     printf("%-25s%s\n", "Stream order:", o.sorted ? "sorted by (region, interval)" : "track");
     border_print();
 

@@ -20,15 +20,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsmoc.so")
 DRIVER_PATH = os.path.join(HERE, "smoc")
 
-OK, ERR_ARG, ERR_NODEV, ERR_HIP, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
+OK, ERR_ARG, ERR_NODEV, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_RANGE = 0, -1, -2, -3, -4, -5, -6
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 EXP_NATIVE, EXP_TABLE, EXP_ACCURATE = 0, 1, 2
-TALLY_ATOMIC_F32, TALLY_ATOMIC_F64, TALLY_DET_F64, TALLY_DET_KAHAN = 0, 1, 2, 3
+TALLY_ATOMIC_F32, TALLY_ATOMIC_F64, TALLY_DET_F64, TALLY_DET_KAHAN, TALLY_FIXED64 = 0, 1, 2, 3, 4
 
 STORAGE = {"f32": STORE_F32, "bf16": STORE_BF16, "f16": STORE_F16}
 EXP = {"native": EXP_NATIVE, "table": EXP_TABLE, "accurate": EXP_ACCURATE}
 TALLY = {"atomic-f32": TALLY_ATOMIC_F32, "atomic-f64": TALLY_ATOMIC_F64,
-         "det-f64": TALLY_DET_F64, "det-kahan": TALLY_DET_KAHAN}
+         "det-f64": TALLY_DET_F64, "det-kahan": TALLY_DET_KAHAN, "fixed64": TALLY_FIXED64}
 
 
 class SmocError(RuntimeError):

@@ -224,3 +224,36 @@ def test_bad_inputs_rejected_on_host(gpu_ready):
         with pytest.raises(smoc.SmocError) as e:
             ctx.run()
         assert e.value.code == smoc.ERR_STATE
+
+
+def test_fixed64_tally_deterministic(gpu_ready, small):
+    print("case fixed64", flush=True)
+    with smoc.Context(small.G, small.R, small.F, tally="fixed64") as ctx:
+        ctx.load(small)
+        ctx.run()
+        f0, p0 = ctx.results()
+        for launch in ({"groups_per_lane": 1}, {"waves_per_block": 8}, {"chunk": 7}):
+            ctx.run(**launch)
+            f1, p1 = ctx.results()
+            if "chunk" not in launch:  # chunking re-associates psi (rounding level)
+                assert np.array_equal(f0.view(np.uint64), f1.view(np.uint64)), launch
+    check_vs_oracle(small, f0, p0)
+
+
+def test_fixed64_sorted_and_init(gpu_ready):
+    prob = smoc.generate(50_000, G=128, R=300, F=5, seed=12, sorted_stream=True)
+    init = np.random.default_rng(3).random((300, 5, 128), dtype=np.float32)
+    flux, psi, _ = run_gpu(prob, "f16", "accurate", "fixed64", init=init)
+    check_vs_oracle(prob, flux, psi, storage="f16", init=init)
+
+
+def test_fixed64_out_of_range_is_reported(gpu_ready):
+    # sigT = 1e-4 makes the 2 ev / sigT^3 term of the flux integral exceed 2^31
+    prob = smoc.generate(2_000, G=64, R=20, F=5, seed=4)
+    prob.sigT = np.full_like(prob.sigT, 1e-4)
+    prob.len = np.full_like(prob.len, 0.1)
+    with smoc.Context(64, 20, 5, tally="fixed64") as ctx:
+        ctx.load(prob)
+        with pytest.raises(smoc.SmocError) as e:
+            ctx.run()
+        assert e.value.code == smoc.ERR_RANGE
