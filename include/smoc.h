@@ -188,6 +188,50 @@ int smoc_set_host_threads(int n);
 double smoc_checksum_f64(const double *x, int64_t n);
 double smoc_checksum_f32(const float *x, int64_t n);
 
+/* ---- fixture files -------------------------------------------------------
+ * Binary files that carry a problem (inputs) or a sweep's results between
+ * hosts, so parity can be checked where the oracle does not run.  Both are
+ * little-endian: a fixed header (magic, version, shape) followed by the raw
+ * arrays, with a 64-bit FNV-1a hash of the arrays in the header.  Readers
+ * check magic, version, sizes and hash and fail with SMOC_ERR_ARG (bad or
+ * corrupt file) or SMOC_ERR_NOMEM/SMOC_ERR_ARG on I/O errors; the message
+ * names the problem. */
+typedef struct {
+    int G, R, F;
+    int64_t S, T;
+    uint64_t seed;   /* generator seed, informational (0 if unknown) */
+    uint32_t flags;  /* bit 0: stream sorted by (region, fine interval) */
+} smoc_problem_info;
+
+typedef struct {
+    int G, R, F;
+    int64_t T;
+    int storage, exp_mode, tally; /* how the results were produced */
+    double fine_flux_checksum;    /* fixed-order FP64 sums */
+    double state_flux_checksum;
+} smoc_results_info;
+
+/* problem file: sigT [R][G], fine_source [R][F][G], len/region/z [S],
+ * track_off [T+1], psi_in [T][G] */
+int smoc_problem_write(const char *path, const smoc_problem_info *info, const float *sigT,
+                       const float *fine_source, const float *len, const int32_t *region,
+                       const float *z, const int64_t *track_off, const float *psi_in);
+int smoc_problem_read_info(const char *path, smoc_problem_info *info);
+int smoc_problem_read(const char *path, float *sigT, float *fine_source, float *len,
+                      int32_t *region, float *z, int64_t *track_off, float *psi_in);
+/* results file: fine_flux [R][F][G] (FP64), psi_out [T][G]; the checksums
+ * are computed by the writer */
+int smoc_results_write(const char *path, const smoc_results_info *info, const double *fine_flux,
+                       const float *psi_out);
+int smoc_results_read_info(const char *path, smoc_results_info *info);
+int smoc_results_read(const char *path, double *fine_flux, float *psi_out);
+/* Compare results against a results file: checksum relative errors and
+ * normwise (l2) relative errors of fine_flux and psi_out (any pointer may be
+ * NULL).  Shapes must match. */
+int smoc_results_compare(const char *path, const double *fine_flux, const float *psi_out,
+                         double *fine_checksum_rel, double *state_checksum_rel,
+                         double *fine_normwise, double *state_normwise);
+
 /* ---- device path -------------------------------------------------------- */
 int smoc_device_count(int *n);
 /* Device name and compute-unit count for reports. */

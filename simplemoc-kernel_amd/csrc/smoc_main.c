@@ -28,6 +28,8 @@ typedef struct {
     int steps;
     int nthreads;
     int p_given;
+    const char *save_problem, *load_problem, *save_results, *check_results;
+    double tol;
     smoc_launch launch;
 } options;
 
@@ -76,6 +78,9 @@ static void print_CLI_error(void)
     printf("  --seed N               generator seed (default 42)\n");
     printf("  --steps K              timed runs (default 1)\n");
     printf("  --gpl 1|2 --wpb W --persistent --graph --autotune   launch tuning\n");
+    printf("  --save-problem F / --load-problem F   write / read the inputs (fixture file)\n");
+    printf("  --save-results F / --check-results F  write results / compare with a results\n");
+    printf("                         file (exit 3 when an error exceeds --tol, default 1e-5)\n");
     printf("See README for the default run values\n");
     exit(1);
 }
@@ -139,6 +144,16 @@ static void read_CLI(int argc, char *argv[], options *o)
             o->launch.use_graph = 1;
         else if (strcmp(arg, "--autotune") == 0)
             o->launch.autotune = 1;
+        else if (strcmp(arg, "--save-problem") == 0)
+            o->save_problem = NEXT();
+        else if (strcmp(arg, "--load-problem") == 0)
+            o->load_problem = NEXT();
+        else if (strcmp(arg, "--save-results") == 0)
+            o->save_results = NEXT();
+        else if (strcmp(arg, "--check-results") == 0)
+            o->check_results = NEXT();
+        else if (strcmp(arg, "--tol") == 0)
+            o->tol = atof(NEXT());
         else
             print_CLI_error();
 #undef NEXT
@@ -166,18 +181,35 @@ int main(int argc, char *argv[])
     o.exp_mode = SMOC_EXP_ACCURATE; /* --exp native: the reference's literal form */
     o.storage = SMOC_STORE_F32;
     o.tally = SMOC_TALLY_ATOMIC_F32;
+    o.tol = 1e-5;
     read_CLI(argc, argv, &o);
     smoc_set_host_threads(o.nthreads);
     smoc_derive_regions(&o.in);
-    const int R = o.regions_override > 0 ? o.regions_override : o.in.source_3D_regions;
-    const int G = o.in.egroups, F = o.in.fine_axial_intervals;
+    int R = o.regions_override > 0 ? o.regions_override : o.in.source_3D_regions;
+    int G = o.in.egroups, F = o.in.fine_axial_intervals;
+    smoc_problem_info pinfo;
+    memset(&pinfo, 0, sizeof(pinfo));
+    if (o.load_problem) {
+        /* the fixture file fixes the problem: its shape overrides -s -e --regions */
+        int rc = smoc_problem_read_info(o.load_problem, &pinfo);
+        if (rc)
+            die("smoc_problem_read_info", rc);
+        G = pinfo.G, R = pinfo.R, F = pinfo.F;
+        o.in.egroups = G, o.in.fine_axial_intervals = F, o.in.segments = pinfo.S;
+        o.seed = pinfo.seed, o.sorted = (int)(pinfo.flags & 1u), o.lognormal_tracks = 0;
+    }
     /* -p: segments per work item (tracks longer than p are split into chunks
      * joined by the psi scan); without -p tracks are swept whole */
     o.launch.seg_per_block = o.p_given ? o.in.seg_per_block : 0;
 
     int64_t S = o.in.segments, T = 0;
     int64_t *track_off = NULL;
-    if (o.lognormal_tracks > 0) {
+    if (o.load_problem) {
+        T = pinfo.T;
+        track_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(T + 1));
+        if (!track_off)
+            die("track allocation", SMOC_ERR_NOMEM);
+    } else if (o.lognormal_tracks > 0) {
         T = o.lognormal_tracks;
         track_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(T + 1));
         if (!track_off)
@@ -236,7 +268,10 @@ int main(int argc, char *argv[])
 
     center_print("INITIALIZATION", 80);
     border_print();
-    printf("Generating seeded inputs (seed %llu)...\n", (unsigned long long)o.seed);
+    if (o.load_problem)
+        printf("Reading inputs from %s...\n", o.load_problem);
+    else
+        printf("Generating seeded inputs (seed %llu)...\n", (unsigned long long)o.seed);
     float *sigT = (float *)malloc(sizeof(float) * (size_t)R * G);
     float *src = (float *)malloc(sizeof(float) * (size_t)R * F * G);
     float *len = (float *)malloc(sizeof(float) * (size_t)(S > 0 ? S : 1));
@@ -246,14 +281,26 @@ int main(int argc, char *argv[])
     if (!sigT || !src || !len || !region || !z || !psi_in)
         die("host allocation", SMOC_ERR_NOMEM);
     int rc;
-    if ((rc = smoc_generate_regions(o.seed, G, R, F, sigT, src)))
-        die("smoc_generate_regions", rc);
-    if ((rc = smoc_generate_segments(o.seed, S, R, len, region, z)))
-        die("smoc_generate_segments", rc);
-    if ((rc = smoc_generate_psi(o.seed, T, G, psi_in)))
-        die("smoc_generate_psi", rc);
-    if (o.sorted && (rc = smoc_sort_segments(S, R, F, len, region, z)))
-        die("smoc_sort_segments", rc);
+    if (o.load_problem) {
+        if ((rc = smoc_problem_read(o.load_problem, sigT, src, len, region, z, track_off, psi_in)))
+            die("smoc_problem_read", rc);
+    } else {
+        if ((rc = smoc_generate_regions(o.seed, G, R, F, sigT, src)))
+            die("smoc_generate_regions", rc);
+        if ((rc = smoc_generate_segments(o.seed, S, R, len, region, z)))
+            die("smoc_generate_segments", rc);
+        if ((rc = smoc_generate_psi(o.seed, T, G, psi_in)))
+            die("smoc_generate_psi", rc);
+        if (o.sorted && (rc = smoc_sort_segments(S, R, F, len, region, z)))
+            die("smoc_sort_segments", rc);
+    }
+    if (o.save_problem) {
+        smoc_problem_info wi = {G, R, F, S, T, o.seed, o.sorted ? 1u : 0u};
+        if ((rc = smoc_problem_write(o.save_problem, &wi, sigT, src, len, region, z, track_off,
+                                     psi_in)))
+            die("smoc_problem_write", rc);
+        printf("Wrote inputs to %s\n", o.save_problem);
+    }
 
     smoc_params p = {o.device, G, R, F, o.storage, o.exp_mode, o.tally};
     smoc_ctx *ctx = NULL;
@@ -308,9 +355,29 @@ int main(int argc, char *argv[])
     printf("%-25s%.17g\n", "fine_flux checksum:", ck_flux);
     printf("%-25s%.17g\n", "state_flux checksum:", ck_psi);
     border_print();
+    int status = 0;
+    if (o.save_results) {
+        smoc_results_info ri = {G, R, F, T, o.storage, o.exp_mode, o.tally, 0.0, 0.0};
+        if ((rc = smoc_results_write(o.save_results, &ri, flux, psi_out)))
+            die("smoc_results_write", rc);
+        printf("Wrote results to %s\n", o.save_results);
+    }
+    if (o.check_results) {
+        double e[4];
+        if ((rc = smoc_results_compare(o.check_results, flux, psi_out, &e[0], &e[1], &e[2],
+                                       &e[3])))
+            die("smoc_results_compare", rc);
+        double worst = 0.0;
+        for (int i = 0; i < 4; i++)
+            worst = e[i] > worst ? e[i] : worst;
+        printf("%-25sfine_flux %.3e, state_flux %.3e (checksums); %.3e, %.3e (normwise)\n",
+               "Check vs results file:", e[0], e[1], e[2], e[3]);
+        status = worst <= o.tol ? 0 : 3;
+        printf("%-25s%s (tolerance %.1e)\n", "Check:", status ? "FAILED" : "passed", o.tol);
+    }
 
     smoc_destroy(ctx);
     free(flux), free(psi_out), free(sigT), free(src), free(len), free(region), free(z),
         free(psi_in), free(track_off);
-    return 0;
+    return status;
 }

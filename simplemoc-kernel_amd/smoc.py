@@ -69,6 +69,19 @@ class Input(ctypes.Structure):
                 ("seg_per_block", ctypes.c_int), ("source_3D_regions", ctypes.c_int)]
 
 
+class ProblemInfo(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_int), ("R", ctypes.c_int), ("F", ctypes.c_int),
+                ("S", ctypes.c_int64), ("T", ctypes.c_int64), ("seed", ctypes.c_uint64),
+                ("flags", ctypes.c_uint32)]
+
+
+class ResultsInfo(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_int), ("R", ctypes.c_int), ("F", ctypes.c_int),
+                ("T", ctypes.c_int64), ("storage", ctypes.c_int), ("exp_mode", ctypes.c_int),
+                ("tally", ctypes.c_int), ("fine_flux_checksum", ctypes.c_double),
+                ("state_flux_checksum", ctypes.c_double)]
+
+
 _F32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _F64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _I32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -96,6 +109,15 @@ SIGNATURES = {
     "smoc_set_host_threads": (_i, [_i]),
     "smoc_checksum_f64": (_d, [_F64, _i64]),
     "smoc_checksum_f32": (_d, [_F32, _i64]),
+    "smoc_problem_write": (_i, [ctypes.c_char_p, ctypes.POINTER(ProblemInfo), _F32, _F32, _P,
+                                _P, _P, _I64, _P]),
+    "smoc_problem_read_info": (_i, [ctypes.c_char_p, ctypes.POINTER(ProblemInfo)]),
+    "smoc_problem_read": (_i, [ctypes.c_char_p, _F32, _F32, _P, _P, _P, _I64, _P]),
+    "smoc_results_write": (_i, [ctypes.c_char_p, ctypes.POINTER(ResultsInfo), _F64, _P]),
+    "smoc_results_read_info": (_i, [ctypes.c_char_p, ctypes.POINTER(ResultsInfo)]),
+    "smoc_results_read": (_i, [ctypes.c_char_p, _F64, _P]),
+    "smoc_results_compare": (_i, [ctypes.c_char_p, _F64, _P, ctypes.POINTER(_d),
+                                  ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "smoc_device_count": (_i, [ctypes.POINTER(_i)]),
     "smoc_device_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i)]),
     "smoc_probe_copy": (_i, [_i, _i64, _i, ctypes.POINTER(_d)]),
@@ -312,3 +334,64 @@ def checksums(flux: np.ndarray, psi: np.ndarray):
     f = np.ascontiguousarray(flux, np.float64).ravel()
     p = np.ascontiguousarray(psi, np.float32).ravel()
     return L.smoc_checksum_f64(f, f.size), L.smoc_checksum_f32(p, p.size)
+
+
+# ---- fixture files ----------------------------------------------------------
+def _ptr(a):
+    return a.ctypes.data if a.size else None
+
+
+def save_problem(path: str, prob: Problem, sorted_stream: bool = False):
+    info = ProblemInfo(prob.G, prob.R, prob.F, prob.S, prob.T, prob.seed, 1 if sorted_stream else 0)
+    arrs = [np.ascontiguousarray(prob.len, np.float32), np.ascontiguousarray(prob.region, np.int32),
+            np.ascontiguousarray(prob.z, np.float32), np.ascontiguousarray(prob.psi_in, np.float32)]
+    _check(lib().smoc_problem_write(os.fsencode(path), ctypes.byref(info),
+                                    np.ascontiguousarray(prob.sigT, np.float32).ravel(),
+                                    np.ascontiguousarray(prob.fine_source, np.float32).ravel(),
+                                    _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                                    np.ascontiguousarray(prob.track_off, np.int64),
+                                    _ptr(arrs[3])), "smoc_problem_write")
+
+
+def load_problem(path: str) -> Problem:
+    info = ProblemInfo()
+    _check(lib().smoc_problem_read_info(os.fsencode(path), ctypes.byref(info)),
+           "smoc_problem_read_info")
+    G, R, F, S, T = info.G, info.R, info.F, info.S, info.T
+    sig, src = np.empty((R, G), np.float32), np.empty((R, F, G), np.float32)
+    ln, rg, zz = np.empty(S, np.float32), np.empty(S, np.int32), np.empty(S, np.float32)
+    off, psi = np.empty(T + 1, np.int64), np.empty((T, G), np.float32)
+    _check(lib().smoc_problem_read(os.fsencode(path), sig.ravel(), src.ravel(), _ptr(ln),
+                                   _ptr(rg), _ptr(zz), off, _ptr(psi)), "smoc_problem_read")
+    return Problem(G, R, F, sig, src, ln, rg, zz, off, psi, int(info.seed))
+
+
+def save_results(path: str, flux, psi, storage="f32", exp="accurate", tally="atomic-f32"):
+    flux = np.ascontiguousarray(flux, np.float64)
+    psi = np.ascontiguousarray(psi, np.float32)
+    R, F, G = flux.shape
+    info = ResultsInfo(G, R, F, psi.shape[0], STORAGE.get(storage, storage), EXP.get(exp, exp),
+                       TALLY.get(tally, tally), 0.0, 0.0)
+    _check(lib().smoc_results_write(os.fsencode(path), ctypes.byref(info), flux.ravel(),
+                                    _ptr(psi)), "smoc_results_write")
+
+
+def load_results(path: str):
+    info = ResultsInfo()
+    _check(lib().smoc_results_read_info(os.fsencode(path), ctypes.byref(info)),
+           "smoc_results_read_info")
+    flux = np.empty((info.R, info.F, info.G), np.float64)
+    psi = np.empty((info.T, info.G), np.float32)
+    _check(lib().smoc_results_read(os.fsencode(path), flux.ravel(), _ptr(psi)),
+           "smoc_results_read")
+    return info, flux, psi
+
+
+def compare_results(path: str, flux, psi):
+    flux = np.ascontiguousarray(flux, np.float64)
+    psi = np.ascontiguousarray(psi, np.float32)
+    out = [_d() for _ in range(4)]
+    _check(lib().smoc_results_compare(os.fsencode(path), flux.ravel(), _ptr(psi),
+                                      *[ctypes.byref(o) for o in out]), "smoc_results_compare")
+    return dict(zip(("fine_checksum_rel", "state_checksum_rel", "fine_normwise",
+                     "state_normwise"), (o.value for o in out)))

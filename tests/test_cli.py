@@ -5,6 +5,7 @@ import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 import smoc
@@ -64,3 +65,24 @@ def test_driver_report_and_checksums_match_api(gpu_ready):
     # deterministic tally: the driver and the API produce identical sums
     assert ck["fine_flux checksum"] == float(f"{f:.17g}")
     assert ck["state_flux checksum"] == float(f"{p:.17g}")
+
+
+@pytest.mark.gpu
+def test_driver_checks_against_oracle_results_file(gpu_ready, tmp_path):
+    """Parity through files: the problem and the FP64 oracle's results are
+    written on the host, the driver loads the problem, sweeps it on the GPU
+    and compares with the results file (exit 3 when over --tol)."""
+    import pyoracle
+    prob = smoc.generate(300_000, G=100, R=6750, F=5, seed=5, track_len=150)
+    ppath, rpath = str(tmp_path / "p.smocprb"), str(tmp_path / "r.smocres")
+    smoc.save_problem(ppath, prob)
+    of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=16)
+    smoc.save_results(rpath, of, op.astype(np.float32), exp="native", tally="det-f64")
+    print("case: driver --load-problem 300k x 100 --check-results (FP64 oracle)", flush=True)
+    r = run("--load-problem", ppath, "--check-results", rpath, "--tally", "det-f64")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "Check:                   passed" in r.stdout
+    assert "Segments:                300,000" in r.stdout
+    # a tolerance below the FP32 rounding of psi_out in the file fails with status 3
+    r = run("--load-problem", ppath, "--check-results", rpath, "--tol", "1e-12")
+    assert r.returncode == 3 and "FAILED" in r.stdout
