@@ -144,7 +144,7 @@ struct smoc_ctx {
     }
     bool fixed() const { return p.tally == SMOC_TALLY_FIXED64; }
     // fine_flux starts 256 bytes into its buffer; with FIXED64 the 8 bytes
-    // just before it count saturated tallies
+    // just before it count FIXED64 contributions beyond the bound (dropped)
     int64_t *fixed_flux() const { return reinterpret_cast<int64_t *>(flux.p + 256); }
     bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
 };
@@ -988,7 +988,8 @@ int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
         uint32_t overflows = 0;
         HIPCHK(hipMemcpy(&overflows, c->flux.p + 248, sizeof(overflows), hipMemcpyDeviceToHost));
         if (overflows) {
-            smoc_set_error("smoc_run: %u FIXED64 tallies left the range |x| < 2^31 and saturated",
+            smoc_set_error("smoc_run: %u FIXED64 run sums exceeded the per-contribution bound "
+                           "and were dropped; fine_flux is incomplete (use an FP64 tally)",
                            overflows);
             return SMOC_ERR_RANGE;
         }
