@@ -64,7 +64,9 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--cpu-sample-segments", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample-segments", type=int, default=16_000_000)
+    ap.add_argument("--cpu-single-segments", type=int, default=1_000_000,
+                    help="sample for the one-thread CPU run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (counter "
                          "calibration) and print its bandwidth")
@@ -95,37 +97,66 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_leg(prob, sample_segments, gpu_plain, mixed, accuracy):
-    """Reference CPU baseline on a bounded sample, and the checker: FP64
-    oracle on the full stream for the accuracy of the GPU results."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
+    """Reference CPU baseline on a bounded sample (all host threads, then one
+    thread on a smaller sample), and the checker: FP64 oracle on the full
+    stream for the accuracy of the GPU results."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = min(threads, os.cpu_count() or threads)
-    T = int(np.searchsorted(prob.track_off, sample_segments, side="right")) - 1
-    T = max(1, min(T, prob.T))
-    S = int(prob.track_off[T])
-    F = prob.F
-    zf = (prob.z[:S] * np.float32(F)).astype(np.float32)
-    fai = np.clip(zf.astype(np.int32), 0, F - 1).astype(np.int32)
-    if pyoracle.ref_available("exp"):
+    use_ref = pyoracle.ref_available("exp")
+    ref = None
+    if use_ref:
         ref = pyoracle.Ref("exp")
         ref.init(prob.G, prob.R, prob.F)
         ref.set_regions(prob.sigT, prob.fine_source)
-        sec, _ = ref.sweep(prob.track_off[: T + 1], prob.region[:S], fai, prob.psi_in[:T],
-                           threads)
+
+    def timed(segments, nthreads):
+        T = int(np.searchsorted(prob.track_off, segments, side="right")) - 1
+        T = max(1, min(T, prob.T))
+        S = int(prob.track_off[T])
+        F = prob.F
+        if use_ref:
+            zf = (prob.z[:S] * np.float32(F)).astype(np.float32)
+            fai = np.clip(zf.astype(np.int32), 0, F - 1).astype(np.int32)
+            sec, _ = ref.sweep(prob.track_off[: T + 1], prob.region[:S], fai, prob.psi_in[:T],
+                               nthreads)
+        else:
+            sub = smoc.Problem(prob.G, prob.R, prob.F, prob.sigT, prob.fine_source,
+                               prob.len[:S], prob.region[:S], prob.z[:S],
+                               prob.track_off[: T + 1], prob.psi_in[:T])
+            sec, _, _ = pyoracle.baseline_f32(sub, pyoracle.EXP_NATIVE, nthreads)
+        return T, S, sec
+
+    if use_ref:
         kind, what = "reference", ("the reference's attenuate_segment (src/cpu/kernel.c, "
                                    "-Ofast -fopenmp, dynamic schedule over tracks, per-(region, "
                                    "interval) locks); ds and zin fixed at its constants")
     else:
-        sub = smoc.Problem(prob.G, prob.R, prob.F, prob.sigT, prob.fine_source, prob.len[:S],
-                           prob.region[:S], prob.z[:S], prob.track_off[: T + 1], prob.psi_in[:T])
-        sec, _, _ = pyoracle.baseline_f32(sub, pyoracle.EXP_NATIVE, threads)
         kind, what = "port", "the C restatement of attenuate_segment, OpenMP over tracks"
+    T, S, sec = timed(sample_segments, threads)
+    T1, S1, sec1 = timed(single_segments, 1)
     base = {"value": S * prob.G / sec, "unit": "segment-group attenuations/s", "cores": threads,
             "kind": kind,
             "sample": f"first {T} tracks ({S} segments x {prob.G} groups) of the benchmark "
-                      f"stream, {what}; {sec:.3f} s wall"}
+                      f"stream, {what}; {sec:.3f} s wall on {threads} threads "
+                      f"({sec * threads:.1f} thread-seconds)",
+            "cpu_model": cpu_model(),
+            "single_thread": {"value": S1 * prob.G / sec1, "cores": 1,
+                              "sample": f"first {T1} tracks ({S1} segments), same code, "
+                                        f"{sec1:.3f} s"}}
     acc = None
     if accuracy:
         acc = {}
@@ -241,7 +272,8 @@ def main():
         mixed = (mprob, mf, mp, mstore)
 
     if not args.no_cpu_baseline:
-        base, acc = cpu_leg(prob, args.cpu_sample_segments, (flux, psi), mixed,
+        base, acc = cpu_leg(prob, args.cpu_sample_segments, args.cpu_single_segments,
+                            (flux, psi), mixed,
                             not args.no_accuracy)
         out["cpu_baseline"] = base
         if acc is not None:

@@ -18,6 +18,9 @@
 
 namespace {
 
+// zeroed entries after the last segment of each device stream array
+constexpr size_t kStreamPad = 64;
+
 #define HIPCHK(call)                                                                           \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \
@@ -104,6 +107,7 @@ struct smoc_ctx {
 
     // region data
     DevBuf<unsigned char> tuples;
+    DevBuf<float> row_thr;  // [R*F] series threshold per row (EXP_ACCURATE)
     DevBuf<float> flux_init;  // [R*F][Gp] or empty (zeros)
     DevBuf<unsigned char> flux;  // f32 or f64 [R*F][Gp]
     DevBuf<float> table;
@@ -113,8 +117,8 @@ struct smoc_ctx {
 
     // segments
     int64_t S = 0, T = 0;
-    DevBuf<float> len, z;
-    DevBuf<int32_t> region;
+    DevBuf<float> len, zin;  // zin: axial offset decoded from z at upload
+    DevBuf<int32_t> code;    // (region * F + f) << 2 | fit kind
     DevBuf<int64_t> track_off;
     DevBuf<float> psi_in, psi_out;  // [T][Gp]
     bool segments_set = false;
@@ -242,9 +246,9 @@ int smoc_create(smoc_ctx **out, const smoc_params *p)
     }
     *out = nullptr;
     if (p->egroups < 1 || p->egroups > 65536 || p->regions < 1 || p->fine_axial < 2 ||
-        (int64_t)p->regions * p->fine_axial > INT32_MAX / 2) {
-        smoc_set_error("smoc_create: need 1 <= egroups <= 65536, regions >= 1, fine_axial >= 2 "
-                       "(got G=%d R=%d F=%d)",
+        (int64_t)p->regions * p->fine_axial > INT32_MAX / 4) {
+        smoc_set_error("smoc_create: need 1 <= egroups <= 65536, regions >= 1, fine_axial >= 2, "
+                       "regions * fine_axial < 2^29 (got G=%d R=%d F=%d)",
                        p->egroups, p->regions, p->fine_axial);
         return SMOC_ERR_ARG;
     }
@@ -314,12 +318,13 @@ int smoc_destroy(smoc_ctx *c)
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     c->tuples.release();
+    c->row_thr.release();
     c->flux_init.release();
     c->flux.release();
     c->table.release();
     c->len.release();
-    c->z.release();
-    c->region.release();
+    c->zin.release();
+    c->code.release();
     c->track_off.release();
     c->psi_in.release();
     c->psi_out.release();
@@ -377,6 +382,8 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
     const int64_t nrow = c->n_fsr();
     int rc = c->tuples.alloc((size_t)nrow * c->Gp * smoc_tuple_bytes(c->p.storage));
     if (rc == SMOC_OK)
+        rc = c->row_thr.alloc((size_t)nrow);
+    if (rc == SMOC_OK)
         rc = c->flux.alloc((size_t)nrow * c->Gp * (c->flux_f64() ? 8 : 4) + 256);
     if (rc != SMOC_OK)
         return rc;
@@ -392,6 +399,8 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
                           c->stream));
     smoc_pack_args pa{dsig.p, dsrc.p, c->tuples.p, G, c->Gp, R, F};
     LAUNCHCHK(smoc_launch_pack(&pa, c->p.storage, c->stream));
+    LAUNCHCHK(smoc_launch_row_thr(c->tuples.p, c->row_thr.p, nrow, G, c->Gp, c->p.storage,
+                                  c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (fine_flux_init) {
         rc = c->flux_init.alloc((size_t)nrow * c->Gp);
@@ -533,11 +542,13 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
         c->fix_limit = (float)(2147483647.0 / most * 0.999);
     }
 
-    int rc = c->len.alloc((size_t)S);
+    // the stream arrays carry kStreamPad zeroed entries past S: the sweep
+    // loads one segment ahead without a bounds branch
+    int rc = c->len.alloc((size_t)S + kStreamPad);
     if (rc == SMOC_OK)
-        rc = c->z.alloc((size_t)S);
+        rc = c->zin.alloc((size_t)S + kStreamPad);
     if (rc == SMOC_OK)
-        rc = c->region.alloc((size_t)S);
+        rc = c->code.alloc((size_t)S + kStreamPad);
     if (rc == SMOC_OK)
         rc = c->track_off.alloc((size_t)T + 1);
     if (rc == SMOC_OK)
@@ -546,10 +557,17 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
         rc = c->psi_out.alloc((size_t)T * c->Gp);
     if (rc != SMOC_OK)
         return rc;
+    HIPCHK(hipMemset(c->len.p + S, 0, sizeof(float) * kStreamPad));
+    HIPCHK(hipMemset(c->zin.p + S, 0, sizeof(float) * kStreamPad));
+    HIPCHK(hipMemset(c->code.p + S, 0, sizeof(int32_t) * kStreamPad));
     if (S > 0) {
         HIPCHK(hipMemcpy(c->len.p, len, sizeof(float) * S, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->z.p, z, sizeof(float) * S, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->region.p, region, sizeof(int32_t) * S, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->zin.p, z, sizeof(float) * S, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->code.p, region, sizeof(int32_t) * S, hipMemcpyHostToDevice));
+        // z -> zin and region -> row code, in place, once per upload
+        HIPCHK((hipError_t)smoc_launch_encode_stream(c->zin.p, c->code.p, c->p.fine_axial, S,
+                                                     c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     HIPCHK(hipMemcpy(c->track_off.p, track_off, sizeof(int64_t) * (T + 1),
                      hipMemcpyHostToDevice));
@@ -698,8 +716,9 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     smoc_sweep_args a{};
     a.tuples = c->tuples.p;
     a.len = c->len.p;
-    a.region = c->region.p;
-    a.z = c->z.p;
+    a.row_thr = c->row_thr.p;
+    a.code = c->code.p;
+    a.zin = c->zin.p;
     a.track_off = c->track_off.p;
     a.psi_in = c->psi_in.p;
     a.psi_out = c->psi_out.p;

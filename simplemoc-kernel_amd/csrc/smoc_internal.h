@@ -28,8 +28,9 @@ void smoc_set_error(const char *fmt, ...);
 typedef struct {
     const void *tuples;          /* packed region data [R*F][Gp] (see pack)   */
     const float *len;            /* [S] */
-    const int32_t *region;       /* [S] */
-    const float *z;              /* [S] */
+    const int32_t *code;         /* [S] (region * F + f) << 2 | fit kind (0 interior,
+                                    1 first, 2 last fine interval) */
+    const float *zin;            /* [S] axial offset inside the fine interval */
     const int64_t *track_off;    /* [T+1] */
     const float *psi_in;         /* [T][Gp] */
     float *psi_out;              /* [T][Gp] */
@@ -38,6 +39,8 @@ typedef struct {
                                     preceding 8 bytes count overflows */
     const int64_t *track_run_off;/* DET_*: first run id of each track [T+1]  */
     const float *table;          /* exp table, 2N floats (EXP_TABLE)          */
+    const float *row_thr;        /* [R*F] EXP_ACCURATE: segments with len below it
+                                    have sigT len < 1 on every group of the row */
     /* chunked sweeps (tracks split into chunks) */
     const int64_t *items;        /* NULL: item i = track i; else [n][4]: track, begin,
                                     end, first run id */
@@ -86,6 +89,8 @@ typedef struct {
 
 /* kernel launchers (smoc_kernels.hip); return 0 or a hipError_t value */
 int smoc_launch_pack(const smoc_pack_args *a, int storage, void *stream);
+int smoc_launch_row_thr(const void *tuples, float *thr, int64_t nrow, int G, int Gp,
+                        int storage, void *stream);
 int smoc_launch_reset(void *flux, const float *init, int64_t n, int f64, void *stream);
 int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl, int exp_mode, int tally,
                       int waves_per_block, int grid_blocks, void *stream);
@@ -93,6 +98,7 @@ int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_
                        void *stream);
 int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream);
 int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
+int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);
 /* bytes per packed tuple for a storage type */

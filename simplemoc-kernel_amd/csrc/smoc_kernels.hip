@@ -9,17 +9,20 @@
 //   * one wavefront sweeps one work item = (track or chunk of a track, group
 //     slice); a slice is 64 * GPL consecutive groups, lane l owning groups
 //     slice*64*GPL + j*64 + l for j < GPL;
-//   * the segment stream (len, region, z: structure of arrays) is read with
-//     wave-uniform addresses, i.e. scalar loads, one segment ahead of the
-//     arithmetic, together with the vector gather of the next segment's
-//     packed region tuple;
+//   * the segment stream is structure of arrays, decoded once at upload
+//     (k_encode_stream) into len, code = (region * F + f) << 2 | fit kind and
+//     the axial offset zin, and read with wave-uniform addresses, i.e. scalar
+//     loads, ahead of the arithmetic together with the vector gather of the
+//     next segment's packed region tuple;
 //   * region data is packed once per smoc_set_regions into one tuple per
 //     (region, fine interval, group): FP32 storage keeps (sigT, c0, c1, c2)
 //     of the source fit, 16-bit storage keeps (sigT, y[f-1], y[f], y[f+1])
 //     and fits in registers;
 //   * consecutive segments of a track in the same (region, interval) form a
 //     run whose tally is summed in registers and flushed once (an atomic,
-//     or a plain store of the run partial in the deterministic modes);
+//     or a plain store of the run partial in the deterministic modes); on a
+//     region-sorted stream the tuple, the source fit and every quantity that
+//     depends only on sigT are computed once per run, not per segment;
 //   * block -> work mapping is XCD-aware: the 8 XCDs each get a contiguous
 //     range of work items (hardware dispatches blocks round-robin over XCDs).
 #include <hip/hip_runtime.h>
@@ -89,58 +92,61 @@ __device__ __forceinline__ float4 unpack(typename Tup<STORE>::raw r)
 
 // ---- segment geometry -----------------------------------------------------
 struct RawSeg {
-    float len, z;
-    int region;
+    float len, zin;
+    int code;  // (region * F + f) << 2 | kind; -1 past the end
 };
+
+// fit kinds (kernel.c:111-191): interior, first and last fine interval
+enum : int { KIND_INTERIOR = 0, KIND_FIRST = 1, KIND_LAST = 2 };
 
 struct Seg {
     float ds;   // segment length
     float zin;  // axial offset inside the fine interval
-    int f;      // fine axial interval
+    int kind;   // KIND_*
     int row;    // region * F + f, -1 past the end of the item
 };
 
 __device__ __forceinline__ RawSeg load_raw(const float *__restrict__ len,
-                                           const int32_t *__restrict__ region,
-                                           const float *__restrict__ z, int64_t k)
+                                           const int32_t *__restrict__ code,
+                                           const float *__restrict__ zin, int64_t k)
 {
-    return RawSeg{len[k], z[k], region[k]};
+    return RawSeg{len[k], zin[k], code[k]};
 }
 
-// Same FP32 operations as smoc_decode_z on the host (no contraction).
-__device__ __forceinline__ Seg decode(const RawSeg r, int F)
+__device__ __forceinline__ Seg decode(const RawSeg r)
+{
+    return Seg{r.len, r.zin, r.code & 3, r.code >> 2};
+}
+
+// z -> (f, zin) with the same FP32 operations as smoc_decode_z on the host
+// (no contraction), so the fine interval and offset are bit-identical.
+__device__ __forceinline__ void decode_z(float z, int F, int &f, float &zin)
 {
 #pragma clang fp contract(off)
-    Seg s;
-    s.ds = r.len;
-    const float zf = r.z * (float)F;
-    int f = (int)zf;
-    f = f > F - 1 ? F - 1 : f;
-    f = f < 0 ? 0 : f;
-    s.f = f;
-    s.zin = ((zf - (float)f) - 0.5f) * kDz;
-    s.row = r.region * F + f;
-    return s;
+    const float zf = z * (float)F;
+    int fi = (int)zf;
+    fi = fi > F - 1 ? F - 1 : fi;
+    fi = fi < 0 ? 0 : fi;
+    f = fi;
+    zin = ((zf - (float)fi) - 0.5f) * kDz;
 }
 
-__device__ __forceinline__ Seg past_end() { return Seg{0.f, 0.f, 0, -1}; }
-
-// Source fit q0, q1, q2 at zin (kernel.c:111-191).
+// Source fit coefficients c0, c1, c2 of one tuple (kernel.c:111-191); the
+// source at zin is q0 = c0 + c1 zin + c2 zin^2, its slope q1 = c1 + 2 c2 zin.
 template <int STORE>
-__device__ __forceinline__ void fit(const float4 t, int f, int F, float zin, float zin2,
-                                    float &q0, float &q1, float &q2)
+__device__ __forceinline__ void fit_coeffs(const float4 t, int kind, float &c0, float &c1,
+                                           float &c2)
 {
-    float c0, c1, c2;
     if constexpr (STORE == SMOC_STORE_F32) {
         c0 = t.y;  // coefficients were fitted by the pack kernel
         c1 = t.z;
         c2 = t.w;
     } else {
         c0 = t.z;
-        if (f == 0) {
+        if (kind == KIND_FIRST) {
             c1 = (t.w - t.z) * kInvDz;
             c2 = 0.f;
-        } else if (f == F - 1) {
+        } else if (kind == KIND_LAST) {
             c1 = (t.z - t.y) * kInvDz;
             c2 = 0.f;
         } else {
@@ -148,6 +154,14 @@ __device__ __forceinline__ void fit(const float4 t, int f, int F, float zin, flo
             c2 = ((t.y - 2.f * t.z) + t.w) * kInv2Dz2;
         }
     }
+}
+
+template <int STORE>
+__device__ __forceinline__ void fit(const float4 t, int kind, float zin, float zin2, float &q0,
+                                    float &q1, float &q2)
+{
+    float c0, c1, c2;
+    fit_coeffs<STORE>(t, kind, c0, c1, c2);
     q0 = c0 + c1 * zin + c2 * zin2;
     q1 = c1 + 2.f * c2 * zin;
     q2 = c2;
@@ -171,7 +185,7 @@ struct EvTerms {
 // poly = 6 tau^4 r4 without subtracting nearly equal numbers.  Degree 6 in
 // r4 truncates at tau^7/11! * 4! < 7e-7 relative to r4 for tau < 1, which
 // moves ev by under 3e-8 absolute.
-__device__ __forceinline__ EvTerms ev_series(float tau)
+__device__ __forceinline__ float r4_series(float tau)
 {
     float r4 = 1.f / 3628800.f;
     r4 = r4 * tau - 1.f / 362880.f;
@@ -179,7 +193,12 @@ __device__ __forceinline__ EvTerms ev_series(float tau)
     r4 = r4 * tau - 1.f / 5040.f;
     r4 = r4 * tau + 1.f / 720.f;
     r4 = r4 * tau - 1.f / 120.f;
-    r4 = r4 * tau + 1.f / 24.f;
+    return r4 * tau + 1.f / 24.f;
+}
+
+__device__ __forceinline__ EvTerms ev_series(float tau)
+{
+    const float r4 = r4_series(tau);
     const float t2 = tau * tau;
     const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
     const float tme = t2 * r;
@@ -372,20 +391,100 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // (its incoming psi is psi_in) and, in the same launch, runs the transfer pass
 // of the middle chunks; phase 2, after the scan, sweeps the remaining chunks.
 // MODE_MIXED kernels pick the mode per work entry from its ITEM_XFER flag.
+
 enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2 };
 enum { ITEM_XFER = 1, ITEM_EMIT = 2 };
+
+// Per-lane constants of one run (a track's consecutive segments in one
+// (region, interval)): everything of kernel.c:111-332 that depends on the
+// region tuple only.  With them a segment costs tau, ev and the two affine
+// updates.
+struct RunK {
+    float sig, i1, i2, i2x2;  // sigT, 1/sigT, 1/sigT^2, 2/sigT^2
+    float c0, c1, c2;         // source fit: q0 = c0 + c1 zin + c2 zin^2
+    float a1, b1;             // mu q1 = a1 + b1 zin
+    float q2m, kfi6;          // mu2 q2 and 2 mu2 q2 / sigT^4 (the integral's q2 term
+                              // is kfi6 * poly / 6)
+};
+
+template <int STORE>
+__device__ __forceinline__ RunK run_consts(const float4 t, int kind)
+{
+    RunK k;
+    k.sig = t.x;
+    k.i1 = rcp(t.x);
+    k.i2 = k.i1 * k.i1;
+    k.i2x2 = 2.f * k.i2;
+    fit_coeffs<STORE>(t, kind, k.c0, k.c1, k.c2);
+    k.a1 = SMOC_MU * k.c1;
+    k.b1 = (2.f * SMOC_MU) * k.c2;
+    k.q2m = SMOC_MU2 * k.c2;
+    k.kfi6 = k.q2m * (k.i2 * k.i2 * 2.f);
+    return k;
+}
+
+// attenuate() with the run constants hoisted: same formulas, the products
+// of constants regrouped (rounding-level differences only).
+template <int EXPM>
+__device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zin, float zin2,
+                                               float &psi, float &one_m_ev, const ExpTab &tb)
+{
+    const float tau = k.sig * ds;
+    const EvTerms x = ev_terms<EXPM>(tau, tb);
+    const float e1 = x.ev * k.i1;
+    const float m2 = x.tme * k.i2;
+    const float reuse = tau * (tau - 2.f) + e1 * k.i2x2;
+    const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
+    const float q1m = k.a1 + k.b1 * zin;
+    const float flux_integral =
+        q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * (x.poly * (1.f / 6.f));
+    psi = q0 * e1 + q1m * m2 + k.q2m * reuse + psi * x.one_m_ev;
+    one_m_ev = x.one_m_ev;
+    return flux_integral;
+}
+
+// attenuate_run() for a segment on which every lane of the wave has tau < 1
+// (ACCURATE exp, checked once per segment on the scalar unit): the series
+// alone, with tau^2 shared by the series and reuse, poly / 6 = tau^4 r4 and
+// psi (1 - ev) as psi - psi ev (exact enough here: 1 - ev >= e^-1).
+template <int MODE>
+__device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float zin, float zin2,
+                                                  float &psi, float &one_m_ev)
+{
+    const float tau = k.sig * ds;
+    const float r4 = r4_series(tau);
+    const float t2 = tau * tau;
+    const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
+    const float tme = t2 * r;  // tau - ev
+    const float ev = tau - tme;
+    const float p6 = (t2 * t2) * r4;
+    const float e1 = ev * k.i1;
+    const float m2 = tme * k.i2;
+    const float reuse = (t2 - 2.f * tau) + e1 * k.i2x2;
+    const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
+    const float q1m = k.a1 + k.b1 * zin;
+    const float flux_integral = q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * p6;
+    if constexpr (MODE == MODE_XFER)
+        one_m_ev = 1.f - ev;
+    float p = psi - psi * ev;  // accumulated so that each term fuses into an FMA
+    p += k.q2m * reuse;
+    p += q1m * m2;
+    psi = p + q0 * e1;
+    return flux_integral;
+}
+
+
 
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                            const float *__restrict__ len,
-                                           const int32_t *__restrict__ region,
-                                           const float *__restrict__ z,
+                                           const int32_t *__restrict__ code,
+                                           const float *__restrict__ zin,
                                            const int64_t *__restrict__ track_off,
                                            const int64_t *__restrict__ items, int64_t item,
                                            int lane, const ExpTab &tb)
 {
     using raw_t = typename Tup<STORE>::raw;
-    const int F = a.F;
     const int64_t w = a.slices == 1 ? item : item / a.slices;
     const int sl = (int)(item - w * a.slices);
     // list entries: item id << 2 | ITEM_* flags
@@ -425,80 +524,149 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
     }
 
     const raw_t *__restrict__ tup = static_cast<const raw_t *>(a.tuples);
-    // Software pipeline over the item's segments: while segment k is
-    // computed from tuple buffer X (or Y), the tuples of k+1 are in flight
-    // into the other buffer and the stream entries of k+2 into scalar
-    // registers.  Two explicit buffers avoid register copies that would force
-    // a wait on the freshly issued gathers.
-    // Gathers and (without RUNS) flushes are unconditional so that the
-    // compiler's memory-counter waits stay exact: a load issued on only one
-    // side of a branch makes every later wait assume the shorter path and
-    // stall on the loads just issued.  Past the end, row 0 is read and unused.
-    // uniform row base + 32-bit lane offset: global_load with an SGPR base
-    auto gather = [&](raw_t (&buf)[GPL], const Seg &sg) {
-        const int64_t row = sg.row >= 0 ? sg.row : 0;
-        const raw_t *__restrict__ base = tup + row * Gp;
+    // segment indices fit 32 bits (S <= 2^31 - 1 is checked on upload), so
+    // the loop bounds are 32-bit scalar compares; k + 2 cannot wrap unsigned
+    const uint32_t kb32 = (uint32_t)kb, ke32 = (uint32_t)ke;
+    auto raw_at = [&](uint32_t k) {
+        return k < ke32 ? load_raw(len, code, zin, k) : RawSeg{0.f, 0.f, -1};
+    };
+    auto flush_all = [&](int row) {
 #pragma unroll
         for (int j = 0; j < GPL; j++)
-            buf[j] = base[(uint32_t)gi[j]];
+            if (g[j] < G)
+                acc[j].flush(a.flux, row, run, Gp, g[j]);
+        ++run;
     };
-    auto raw_at = [&](int64_t k) {
-        return k < ke ? load_raw(len, region, z, k) : RawSeg{0.f, 0.f, -1};
-    };
-    auto step = [&](const raw_t (&buf)[GPL], const Seg &cur, const Seg &nxt) {
-        const float zin2 = cur.zin * cur.zin;
+
+    if constexpr (!RUNS) {
+        // Unsorted stream: every segment is its own run.  Software pipeline
+        // over the item's segments: while segment k is computed from tuple
+        // buffer X (or Y), the tuples of k+1 are in flight into the other
+        // buffer and the stream entries of k+2 into scalar registers.  Two
+        // explicit buffers avoid register copies that would force a wait on
+        // the freshly issued gathers.  Gathers and flushes are unconditional
+        // so that the compiler's memory-counter waits stay exact: a load
+        // issued on only one side of a branch makes every later wait assume
+        // the shorter path and stall on the loads just issued.  Past the end,
+        // row 0 is read and unused.
+        auto gather = [&](raw_t (&buf)[GPL], const Seg &sg) {
+            const int64_t row = sg.row >= 0 ? sg.row : 0;
+            const raw_t *__restrict__ base = tup + row * Gp;  // uniform base + lane offset
 #pragma unroll
-        for (int j = 0; j < GPL; j++) {
-            const float4 r = unpack<STORE>(buf[j]);
-            float q0, q1, q2, m;
-            fit<STORE>(r, cur.f, F, cur.zin, zin2, q0, q1, q2);
-            const float tally = attenuate<EXPM>(r.x, q0, q1, q2, cur.ds, psi[j], m, tb);
-            if constexpr (MODE == MODE_XFER)
-                mprod[j] *= m;
-            else
-                acc[j].add(tally);
-        }
-        if constexpr (MODE == MODE_SWEEP) {
-            // RUNS: the stream is sorted enough that a track stays in one
-            // (region, interval) for several segments; sum the run in
-            // registers and flush when it ends.  Otherwise every segment is
-            // its own run.
-            if (!RUNS || nxt.row != cur.row) {
+            for (int j = 0; j < GPL; j++)
+                buf[j] = base[(uint32_t)gi[j]];
+        };
+        auto step = [&](const raw_t (&buf)[GPL], const Seg &cur) {
+            const float zin2 = cur.zin * cur.zin;
 #pragma unroll
-                for (int j = 0; j < GPL; j++)
-                    if (g[j] < G)
-                        acc[j].flush(a.flux, cur.row, run, Gp, g[j]);
-                ++run;
+            for (int j = 0; j < GPL; j++) {
+                const float4 r = unpack<STORE>(buf[j]);
+                float q0, q1, q2, m;
+                fit<STORE>(r, cur.kind, cur.zin, zin2, q0, q1, q2);
+                const float tally = attenuate<EXPM>(r.x, q0, q1, q2, cur.ds, psi[j], m, tb);
+                if constexpr (MODE == MODE_XFER)
+                    mprod[j] *= m;
+                else
+                    acc[j].add(tally);
+            }
+            if constexpr (MODE == MODE_SWEEP)
+                flush_all(cur.row);
+        };
+        if (kb32 < ke32) {
+            raw_t X[GPL], Y[GPL];
+#pragma unroll
+            for (int j = 0; j < GPL; j++)
+                X[j] = Y[j] = Tup<STORE>::one();
+            Seg s0 = decode(raw_at(kb32));
+            gather(X, s0);
+            Seg s1 = decode(raw_at(kb32 + 1));
+            for (uint32_t k = kb32;;) {
+                gather(Y, s1);
+                const RawSeg r2 = raw_at(k + 2);
+                step(X, s0);
+                const Seg s2 = decode(r2);
+                if (++k >= ke32)
+                    break;
+                gather(X, s2);
+                const RawSeg r3 = raw_at(k + 2);
+                step(Y, s1);
+                const Seg s3 = decode(r3);
+                if (++k >= ke32)
+                    break;
+                s0 = s2;
+                s1 = s3;
             }
         }
-    };
-    auto dec = [&](const RawSeg &r) { return r.region >= 0 ? decode(r, F) : past_end(); };
-
-    if (kb < ke) {
-        raw_t X[GPL], Y[GPL];
+    } else {
+        // Region-sorted stream: a track stays in one (region, interval) for
+        // many segments.  The outer loop starts a run: it gathers the tuple,
+        // derives the run constants and, at the end, flushes the run's tally
+        // summed in registers.  The inner loop walks the run with as little
+        // scalar work per segment as possible: one scalar load per stream
+        // array, issued a segment ahead (the device arrays are padded, so the
+        // load at the item's end is in bounds and only marked past-the-end),
+        // one compare for the exp branch and one for the run's end.
+        if (kb32 < ke32) {
+            RunK rk[GPL];
+            uint32_t k = kb32;
+            RawSeg cur = load_raw(len, code, zin, k);
+            for (;;) {
+                const int ccode = cur.code;
+                {
+                    const raw_t *__restrict__ base = tup + (int64_t)(ccode >> 2) * Gp;
+                    raw_t b[GPL];
 #pragma unroll
-        for (int j = 0; j < GPL; j++)
-            X[j] = Y[j] = Tup<STORE>::one();
-        Seg s0 = dec(raw_at(kb));
-        gather(X, s0);
-        Seg s1 = dec(raw_at(kb + 1));
-        for (int64_t k = kb;;) {
-            gather(Y, s1);
-            const RawSeg r2 = raw_at(k + 2);
-            step(X, s0, s1);
-            const Seg s2 = dec(r2);
-            if (++k >= ke)
-                break;
-            gather(X, s2);
-            const RawSeg r3 = raw_at(k + 2);
-            step(Y, s1, s2);
-            const Seg s3 = dec(r3);
-            if (++k >= ke)
-                break;
-            s0 = s2;
-            s1 = s3;
+                    for (int j = 0; j < GPL; j++)
+                        b[j] = base[(uint32_t)gi[j]];
+#pragma unroll
+                    for (int j = 0; j < GPL; j++)
+                        rk[j] = run_consts<STORE>(unpack<STORE>(b[j]), ccode & 3);
+                }
+                // per-row threshold: the same for every launch shape, so the
+                // branch taken (and the rounding) does not depend on which
+                // groups share the wave
+                const uint32_t thr =
+                    EXPM == SMOC_EXP_ACCURATE ? __float_as_uint(a.row_thr[ccode >> 2]) : 0u;
+                for (;;) {
+                    RawSeg nx = load_raw(len, code, zin, k + 1);
+                    nx.code = k + 1 < ke32 ? nx.code : -1;
+                    const float zin2 = cur.zin * cur.zin;
+                    if (EXPM == SMOC_EXP_ACCURATE && __float_as_uint(cur.len) < thr) {
+#pragma unroll
+                        for (int j = 0; j < GPL; j++) {
+                            float m;
+                            const float tally =
+                                attenuate_series<MODE>(rk[j], cur.len, cur.zin, zin2, psi[j], m);
+                            if constexpr (MODE == MODE_XFER)
+                                mprod[j] *= m;
+                            else
+                                acc[j].add(tally);
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < GPL; j++) {
+                            float m;
+                            const float tally = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
+                                                                    zin2, psi[j], m, tb);
+                            if constexpr (MODE == MODE_XFER)
+                                mprod[j] *= m;
+                            else
+                                acc[j].add(tally);
+                        }
+                    }
+                    ++k;
+                    cur = nx;
+                    if (cur.code != ccode)
+                        break;
+                }
+                if constexpr (MODE == MODE_SWEEP)
+                    flush_all(ccode >> 2);
+                if (k >= ke32)
+                    break;
+            }
         }
     }
+
     if (MODE == MODE_XFER || emit_xfer) {
         // transfer of this chunk; a swept first chunk knows its psi_out
         // exactly and passes it on as B with M = 0
@@ -520,8 +688,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __device__ __forceinline__ void dispatch_item(const smoc_sweep_args &a,
                                               const float *__restrict__ len,
-                                              const int32_t *__restrict__ region,
-                                              const float *__restrict__ z,
+                                              const int32_t *__restrict__ code,
+                                              const float *__restrict__ zin,
                                               const int64_t *__restrict__ track_off,
                                               const int64_t *__restrict__ items, int64_t item,
                                               int lane, const ExpTab &tb)
@@ -529,13 +697,13 @@ __device__ __forceinline__ void dispatch_item(const smoc_sweep_args &a,
     if constexpr (MODE == MODE_MIXED) {
         const int64_t w = a.slices == 1 ? item : item / a.slices;
         if (uniform64(a.item_list[w]) & ITEM_XFER)
-            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_XFER>(a, len, region, z, track_off,
+            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_XFER>(a, len, code, zin, track_off,
                                                                  items, item, lane, tb);
         else
-            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, len, region, z, track_off,
+            sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, len, code, zin, track_off,
                                                                   items, item, lane, tb);
     } else {
-        sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
+        sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off, items,
                                                         item, lane, tb);
     }
 }
@@ -560,8 +728,8 @@ __device__ __forceinline__ ExpTab stage_table(const smoc_sweep_args &a, float *s
 // One wave per work item, XCD-aware block order.
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __global__ void __launch_bounds__(1024)
-    k_sweep(smoc_sweep_args a, const float *__restrict__ len, const int32_t *__restrict__ region,
-            const float *__restrict__ z, const int64_t *__restrict__ track_off,
+    k_sweep(smoc_sweep_args a, const float *__restrict__ len, const int32_t *__restrict__ code,
+            const float *__restrict__ zin, const int64_t *__restrict__ track_off,
             const int64_t *__restrict__ items)
 {
     __shared__ float s_tab[EXPM == SMOC_EXP_TABLE ? SMOC_TABLE_CAP : 1];
@@ -572,7 +740,7 @@ __global__ void __launch_bounds__(1024)
     const int64_t item = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * wpb + wave;
     if (item >= a.n_items * a.slices)
         return;
-    dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items, item,
+    dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off, items, item,
                                                        lane, tb);
 }
 
@@ -581,7 +749,7 @@ __global__ void __launch_bounds__(1024)
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __global__ void __launch_bounds__(1024)
     k_sweep_persistent(smoc_sweep_args a, const float *__restrict__ len,
-                       const int32_t *__restrict__ region, const float *__restrict__ z,
+                       const int32_t *__restrict__ code, const float *__restrict__ zin,
                        const int64_t *__restrict__ track_off, const int64_t *__restrict__ items)
 {
     __shared__ float s_tab[EXPM == SMOC_EXP_TABLE ? SMOC_TABLE_CAP : 1];
@@ -595,7 +763,7 @@ __global__ void __launch_bounds__(1024)
         const int64_t item = uniform64((int64_t)__shfl(got, 0));
         if (item >= n_work)
             break;
-        dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, region, z, track_off, items,
+        dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off, items,
                                                            item, lane, tb);
     }
 }
@@ -671,6 +839,49 @@ __global__ void __launch_bounds__(256) k_pack(smoc_pack_args a)
         reinterpret_cast<uint2 *>(a.tuples)[i] =
             make_uint2(f_to_h(sig) | (f_to_h(ylo) << 16), f_to_h(ymid) | (f_to_h(yhi) << 16));
     }
+}
+
+// Series threshold of each row (region, interval), for attenuate_series: with
+// ds below it, sigT ds < 1 for every group of the row.  thr = (1 - 2^-20) /
+// max_g sigT (IEEE division, sigT as the sweep unpacks it) gives sigT ds <
+// (1 - 2^-20)(1 + 2^-24) < 1 - 2^-21 before rounding, so the FP32 product
+// stays below 1.  The sweep compares float bits as unsigned integers, which
+// orders non-negative floats numerically.  One wave per row.
+template <int STORE>
+__global__ void __launch_bounds__(256) k_row_thr(const void *tuples, float *thr, int64_t nrow,
+                                                 int G, int Gp)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= nrow)
+        return;
+    using raw_t = typename Tup<STORE>::raw;
+    const raw_t *t = static_cast<const raw_t *>(tuples) + row * Gp;
+    float m = 0.f;
+    for (int g = lane; g < G; g += 64)
+        m = fmaxf(m, unpack<STORE>(t[g]).x);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o));
+    if (lane == 0)
+        thr[row] = (1.f - 0x1p-20f) / m;
+}
+
+// ---- segment stream encoding (once per smoc_set_segments) -----------------------
+// In place: zin_io holds z on entry and zin on exit; code_io holds the region
+// id on entry and (region * F + f) << 2 | fit kind on exit.
+__global__ void __launch_bounds__(256) k_encode_stream(float *zin_io, int32_t *code_io, int F,
+                                                       int64_t S)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S)
+        return;
+    int f;
+    float zi;
+    decode_z(zin_io[k], F, f, zi);
+    const int kind = f == 0 ? KIND_FIRST : (f == F - 1 ? KIND_LAST : KIND_INTERIOR);
+    zin_io[k] = zi;
+    code_io[k] = ((code_io[k] * F + f) << 2) | kind;
 }
 
 // ---- resets --------------------------------------------------------------------
@@ -788,11 +999,11 @@ int launch_sweep_m(const smoc_sweep_args *a, int wpb, int grid, hipStream_t s)
 {
     if (a->queue)
         hipLaunchKernelGGL((k_sweep_persistent<STORE, GPL, EXPM, TALLY, RUNS, MODE>), dim3(grid),
-                           dim3(64 * wpb), 0, s, *a, a->len, a->region, a->z, a->track_off,
+                           dim3(64 * wpb), 0, s, *a, a->len, a->code, a->zin, a->track_off,
                            a->items);
     else
         hipLaunchKernelGGL((k_sweep<STORE, GPL, EXPM, TALLY, RUNS, MODE>), dim3(grid),
-                           dim3(64 * wpb), 0, s, *a, a->len, a->region, a->z, a->track_off,
+                           dim3(64 * wpb), 0, s, *a, a->len, a->code, a->zin, a->track_off,
                            a->items);
     return (int)hipGetLastError();
 }
@@ -880,6 +1091,22 @@ extern "C" int smoc_launch_pack(const smoc_pack_args *a, int storage, void *stre
     return (int)hipGetLastError();
 }
 
+extern "C" int smoc_launch_row_thr(const void *tuples, float *thr, int64_t nrow, int G, int Gp,
+                                   int storage, void *stream)
+{
+    if (nrow == 0)
+        return 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = blocks_for(nrow, 4);
+    switch (storage) {
+    case SMOC_STORE_F32: hipLaunchKernelGGL(k_row_thr<SMOC_STORE_F32>, dim3(nb), dim3(256), 0, s, tuples, thr, nrow, G, Gp); break;
+    case SMOC_STORE_BF16: hipLaunchKernelGGL(k_row_thr<SMOC_STORE_BF16>, dim3(nb), dim3(256), 0, s, tuples, thr, nrow, G, Gp); break;
+    case SMOC_STORE_F16: hipLaunchKernelGGL(k_row_thr<SMOC_STORE_F16>, dim3(nb), dim3(256), 0, s, tuples, thr, nrow, G, Gp); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
 extern "C" int smoc_launch_reset(void *flux, const float *init, int64_t n, int f64, void *stream)
 {
     if (n == 0)
@@ -899,6 +1126,16 @@ extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int gri
         return 0;
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S,
+                                         void *stream)
+{
+    if (S <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_encode_stream, dim3(blocks_for(S, 256)), dim3(256), 0,
+                       (hipStream_t)stream, zin_io, code_io, F, S);
     return (int)hipGetLastError();
 }
 

@@ -257,3 +257,59 @@ def test_fixed64_out_of_range_is_reported(gpu_ready):
         with pytest.raises(smoc.SmocError) as e:
             ctx.run()
         assert e.value.code == smoc.ERR_RANGE
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("exp", ["accurate", "native", "table"])
+def test_sorted_stream_both_tau_regimes(gpu_ready, storage, exp):
+    """Region-sorted stream with lengths spread so that tau = sigT len runs
+    from 0 to ~30: segments on which every lane has tau < 1 take the series
+    branch of the run loop, the others the per-lane exp branch."""
+    prob = smoc.generate(40_000, G=100, R=400, F=5, seed=21, sorted_stream=True)
+    prob.len = (prob.len * np.where(np.arange(prob.S) % 3 == 0, 30.0, 1.0)).astype(np.float32)
+    tau = prob.sigT[prob.region].max(axis=1) * prob.len
+    assert (tau < 1).mean() > 0.3 and (tau >= 1).mean() > 0.2
+    print(f"case sorted both tau regimes {storage} {exp}", flush=True)
+    for tally in ("det-f64", "atomic-f32"):
+        flux, psi, _ = run_gpu(prob, storage, exp, tally)
+        check_vs_oracle(prob, flux, psi, exp=exp, storage=storage)
+
+
+def test_sorted_stream_chunked_transfer(gpu_ready):
+    """Sorted stream, atomic tally, long tracks split into chunks: the
+    transfer pass runs the run loop too."""
+    prob = smoc.generate(60_000, G=100, R=300, F=5, seed=22, sorted_stream=True,
+                         track_len=3000)
+    print("case sorted chunked transfer", flush=True)
+    for chunk in (0, 700):
+        flux, psi, _ = run_gpu(prob, "bf16", "accurate", "atomic-f32", chunk=chunk)
+        check_vs_oracle(prob, flux, psi, storage="bf16")
+
+
+@pytest.mark.parametrize("exp", ["native", "accurate"])
+def test_exp_error_bound(gpu_ready, exp):
+    """The device exponential against FP64 exp on tau = fl(sigT * len) itself.
+
+    Zero source and psi_in = 1 make psi_out = e^-tau as the sweep computes it
+    (kernel.c:289-332 reduces to psi (1 - ev)).  Native mode evaluates
+    2^(-tau log2 e) with the hardware v_exp_f32: its error is the rounding of
+    the argument (relative tau * 2^-24 on the result) plus about 1 ulp.
+    Accurate mode uses the series for tau < 1 (1 - ev rounds once more) and
+    the same exp above.  Bound: 2^-21 + tau * 2^-23 relative (a few ulp plus
+    the argument rounding)."""
+    G, T = 64, 4096
+    sig = np.geomspace(0.01, 20.0, G).astype(np.float32)[None, :]
+    ln = np.linspace(0.0, 2.0, T).astype(np.float32)
+    prob = smoc.Problem(G, 1, 2, sig, np.zeros((1, 2, G), np.float32), ln,
+                        np.zeros(T, np.int32), np.full(T, 0.25, np.float32),
+                        np.arange(T + 1, dtype=np.int64), np.ones((T, G), np.float32))
+    _, psi, _ = run_gpu(prob, "f32", exp, "atomic-f32")
+    tau = (sig * ln[:, None]).astype(np.float32)  # one FP32 rounding, as on the device
+    want = np.exp(-tau.astype(np.float64))
+    relerr = np.abs(psi.astype(np.float64) - want) / want
+    bound = 2.0 ** -21 + tau.astype(np.float64) * 2.0 ** -23
+    worst = np.argmax(relerr / bound)
+    print(f"  {exp}: max rel err {relerr.max():.3e}; worst vs bound at tau "
+          f"{tau.ravel()[worst]:.4g}: {relerr.ravel()[worst]:.3e} <= {bound.ravel()[worst]:.3e}",
+          flush=True)
+    assert np.all(relerr <= bound)
