@@ -445,8 +445,7 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 
 // attenuate_run() for a segment on which every lane of the wave has tau < 1
 // (ACCURATE exp, checked once per segment on the scalar unit): the series
-// alone, with tau^2 shared by the series and reuse, poly / 6 = tau^4 r4 and
-// psi (1 - ev) as psi - psi ev (exact enough here: 1 - ev >= e^-1).
+// alone, with tau^2 shared by the series and reuse, and poly / 6 = tau^4 r4.
 template <int MODE>
 __device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float zin, float zin2,
                                                   float &psi, float &one_m_ev)
@@ -466,10 +465,12 @@ __device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float
     const float flux_integral = q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * p6;
     if constexpr (MODE == MODE_XFER)
         one_m_ev = 1.f - ev;
-    float p = psi - psi * ev;  // accumulated so that each term fuses into an FMA
-    p += k.q2m * reuse;
-    p += q1m * m2;
-    psi = p + q0 * e1;
+    // psi' = S + psi (1 - ev) with S the source terms: the small change
+    // S - psi ev is formed first (one FMA), then added to psi, so only one
+    // rounding happens at the magnitude of psi (a run near its steady state
+    // psi = S / ev would otherwise accumulate the roundings of every term)
+    const float src = q0 * e1 + q1m * m2 + k.q2m * reuse;
+    psi = psi + (src - psi * ev);
     return flux_integral;
 }
 

@@ -313,3 +313,18 @@ def test_exp_error_bound(gpu_ready, exp):
           f"{tau.ravel()[worst]:.4g}: {relerr.ravel()[worst]:.3e} <= {bound.ravel()[worst]:.3e}",
           flush=True)
     assert np.all(relerr <= bound)
+
+
+@pytest.mark.parametrize("tally", ["det-f64", "atomic-f64"])
+def test_sorted_long_runs_small_tau(gpu_ready, tally):
+    """Runs of ~2000 segments with tau ~ 1e-3: psi sits near its steady state
+    S / ev and every rounding made at the magnitude of psi is amplified by
+    about 1 / ev, so an update that rounds more than once at that magnitude
+    drifts measurably.  Same bar as every accurate-mode case."""
+    prob = smoc.generate(200_000, G=100, R=20, F=2, seed=23, sorted_stream=True,
+                         track_len=2000)
+    prob.len = (prob.len * np.float32(0.01)).astype(np.float32)
+    print(f"case sorted long runs small tau {tally}", flush=True)
+    flux, psi, _ = run_gpu(prob, "f32", "accurate", tally)
+    errs = check_vs_oracle(prob, flux, psi)
+    print(f"  errors {errs}", flush=True)
