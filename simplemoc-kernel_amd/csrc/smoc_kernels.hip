@@ -44,6 +44,10 @@ constexpr float kInv2Dz2 = 1.0f / (2.f * kDz * kDz);
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// s_waitcnt immediate (gfx9 encoding) that waits for lgkmcnt = 0 only:
+// vmcnt 63 (bits 3:0 and 15:14), expcnt 7 (bits 6:4), lgkmcnt 0 (bits 11:8)
+constexpr int kWaitLgkm0 = 0xC07F;
+
 // ---- storage formats -------------------------------------------------------
 template <int STORE> struct Tup;
 template <> struct Tup<SMOC_STORE_F32> {
@@ -446,7 +450,10 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 // attenuate_run() for a segment on which every lane of the wave has tau < 1
 // (ACCURATE exp, checked once per segment on the scalar unit): the series
 // alone, with tau^2 shared by the series and reuse, and poly / 6 = tau^4 r4.
-template <int MODE>
+// HAS_C2 = false: a first or last fine interval, whose fit is linear (c2 =
+// 0, so q2 = 0 and q1 = c1): the terms in c2 are exactly zero and skipped,
+// with the same results.
+template <int MODE, bool HAS_C2>
 __device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float zin, float zin2,
                                                   float &psi, float &one_m_ev)
 {
@@ -456,20 +463,27 @@ __device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float
     const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
     const float tme = t2 * r;  // tau - ev
     const float ev = tau - tme;
-    const float p6 = (t2 * t2) * r4;
     const float e1 = ev * k.i1;
     const float m2 = tme * k.i2;
     const float reuse = (t2 - 2.f * tau) + e1 * k.i2x2;
-    const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
-    const float q1m = k.a1 + k.b1 * zin;
-    const float flux_integral = q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * p6;
+    float flux_integral, src;
+    if constexpr (HAS_C2) {
+        const float p6 = (t2 * t2) * r4;
+        const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
+        const float q1m = k.a1 + k.b1 * zin;
+        flux_integral = q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * p6;
+        src = q0 * e1 + q1m * m2 + k.q2m * reuse;
+    } else {
+        const float q0 = k.c0 + k.c1 * zin;
+        flux_integral = q0 * m2 + psi * e1 + k.a1 * reuse;
+        src = q0 * e1 + k.a1 * m2;
+    }
     if constexpr (MODE == MODE_XFER)
         one_m_ev = 1.f - ev;
     // psi' = S + psi (1 - ev) with S the source terms: the small change
     // S - psi ev is formed first (one FMA), then added to psi, so only one
     // rounding happens at the magnitude of psi (a run near its steady state
     // psi = S / ev would otherwise accumulate the roundings of every term)
-    const float src = q0 * e1 + q1m * m2 + k.q2m * reuse;
     psi = psi + (src - psi * ev);
     return flux_integral;
 }
@@ -628,38 +642,54 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                 // groups share the wave
                 const uint32_t thr =
                     EXPM == SMOC_EXP_ACCURATE ? __float_as_uint(a.row_thr[ccode >> 2]) : 0u;
-                for (;;) {
-                    RawSeg nx = load_raw(len, code, zin, k + 1);
-                    nx.code = k + 1 < ke32 ? nx.code : -1;
-                    const float zin2 = cur.zin * cur.zin;
-                    if (EXPM == SMOC_EXP_ACCURATE && __float_as_uint(cur.len) < thr) {
+                // Scalar loads may return out of order, so a wait for any of
+                // them is lgkmcnt(0).  Complete the threshold's load here:
+                // pending on entry to the segment loop, it would make every
+                // iteration wait for the loads just issued for the next
+                // segment, defeating the prefetch.
+                __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+                // the run's segments; the fit kind picks the specialization
+                // once per run (a linear fit drops its zero c2 terms)
+                auto walk = [&](auto has_c2) {
+                    constexpr bool HAS_C2 = decltype(has_c2)::value;
+                    for (;;) {
+                        RawSeg nx = load_raw(len, code, zin, k + 1);
+                        nx.code = k + 1 < ke32 ? nx.code : -1;
+                        const float zin2 = HAS_C2 ? cur.zin * cur.zin : 0.f;
+                        if (EXPM == SMOC_EXP_ACCURATE && __float_as_uint(cur.len) < thr) {
 #pragma unroll
-                        for (int j = 0; j < GPL; j++) {
-                            float m;
-                            const float tally =
-                                attenuate_series<MODE>(rk[j], cur.len, cur.zin, zin2, psi[j], m);
-                            if constexpr (MODE == MODE_XFER)
-                                mprod[j] *= m;
-                            else
-                                acc[j].add(tally);
-                        }
-                    } else {
+                            for (int j = 0; j < GPL; j++) {
+                                float m;
+                                const float tally = attenuate_series<MODE, HAS_C2>(
+                                    rk[j], cur.len, cur.zin, zin2, psi[j], m);
+                                if constexpr (MODE == MODE_XFER)
+                                    mprod[j] *= m;
+                                else
+                                    acc[j].add(tally);
+                            }
+                        } else {
+                            const float zin2g = cur.zin * cur.zin;
 #pragma unroll
-                        for (int j = 0; j < GPL; j++) {
-                            float m;
-                            const float tally = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
-                                                                    zin2, psi[j], m, tb);
-                            if constexpr (MODE == MODE_XFER)
-                                mprod[j] *= m;
-                            else
-                                acc[j].add(tally);
+                            for (int j = 0; j < GPL; j++) {
+                                float m;
+                                const float tally = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
+                                                                        zin2g, psi[j], m, tb);
+                                if constexpr (MODE == MODE_XFER)
+                                    mprod[j] *= m;
+                                else
+                                    acc[j].add(tally);
+                            }
                         }
+                        ++k;
+                        cur = nx;
+                        if (cur.code != ccode)
+                            break;
                     }
-                    ++k;
-                    cur = nx;
-                    if (cur.code != ccode)
-                        break;
-                }
+                };
+                if ((ccode & 3) == KIND_INTERIOR)
+                    walk(std::true_type{});
+                else
+                    walk(std::false_type{});
                 if constexpr (MODE == MODE_SWEEP)
                     flush_all(ccode >> 2);
                 if (k >= ke32)

@@ -328,3 +328,34 @@ def test_sorted_long_runs_small_tau(gpu_ready, tally):
     flux, psi, _ = run_gpu(prob, "f32", "accurate", tally)
     errs = check_vs_oracle(prob, flux, psi)
     print(f"  errors {errs}", flush=True)
+
+
+def test_contexts_in_parallel_threads(gpu_ready):
+    """One context per host thread, each with its own stream: two threads
+    sweeping different problems at the same time get exactly what each gets
+    alone (deterministic tally, so bit for bit)."""
+    import threading
+    probs = [smoc.generate(60_000, G=100, R=500, F=5, seed=s) for s in (31, 32)]
+    alone = [run_gpu(p, "f32", "accurate", "det-f64")[:2] for p in probs]
+    out = [None, None]
+    errs = []
+
+    def work(i):
+        try:
+            with smoc.Context(100, 500, 5, tally="det-f64") as ctx:
+                ctx.load(probs[i])
+                for _ in range(3):
+                    ctx.run()
+                out[i] = ctx.results()
+        except Exception as e:  # reported by the main thread
+            errs.append(e)
+
+    print("case two contexts in two threads", flush=True)
+    th = [threading.Thread(target=work, args=(i,)) for i in (0, 1)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not errs, errs
+    for (f, p), (fa, pa) in zip(out, alone):
+        assert np.array_equal(f, fa) and np.array_equal(p, pa)
