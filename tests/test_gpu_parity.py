@@ -359,3 +359,20 @@ def test_contexts_in_parallel_threads(gpu_ready):
     assert not errs, errs
     for (f, p), (fa, pa) in zip(out, alone):
         assert np.array_equal(f, fa) and np.array_equal(p, pa)
+
+
+@pytest.mark.parametrize("G,F", [(1, 5), (37, 3), (129, 5), (200, 2), (100, 8)])
+def test_sorted_group_and_interval_counts(gpu_ready, G, F):
+    """The run loop of region-sorted streams across group counts (several
+    slices, padding lanes) and fine-interval counts (F = 2: only linear-fit
+    runs), both groups-per-lane settings, and bit-exact deterministic results
+    across the two launch shapes."""
+    prob = smoc.generate(20_000, G=G, R=60, F=F, seed=100 + G + F, sorted_stream=True)
+    print(f"case sorted G={G} F={F}", flush=True)
+    res = []
+    for gpl, wpb in ((1, 4), (2, 16)):
+        flux, psi, _ = run_gpu(prob, "bf16", "accurate", "det-f64", groups_per_lane=gpl,
+                               waves_per_block=wpb)
+        check_vs_oracle(prob, flux, psi, storage="bf16")
+        res.append((flux, psi))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
