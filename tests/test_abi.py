@@ -42,7 +42,9 @@ def test_abi_version_and_errors():
 
 
 def test_create_rejects_bad_params_before_touching_a_device():
-    for kw in ({"G": 0, "R": 10, "F": 5}, {"G": 10, "R": 0, "F": 5}, {"G": 10, "R": 10, "F": 1}):
+    for kw in ({"G": 0, "R": 10, "F": 5}, {"G": 10, "R": 0, "F": 5}, {"G": 10, "R": 10, "F": 1},
+               {"G": 65537, "R": 10, "F": 5},
+               {"G": 10, "R": 1 << 27, "F": 4}):  # R * F = 2^29: row codes need R * F < 2^29
         with pytest.raises(smoc.SmocError) as e:
             smoc.Context(kw["G"], kw["R"], kw["F"])
         assert e.value.code == smoc.ERR_ARG

@@ -376,3 +376,16 @@ def test_sorted_group_and_interval_counts(gpu_ready, G, F):
         check_vs_oracle(prob, flux, psi, storage="bf16")
         res.append((flux, psi))
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_segment_count_limit(gpu_ready):
+    """Segment indices are 32-bit: S >= 2^31 is refused before any array is
+    read (the tiny arrays passed here are never touched)."""
+    prob = smoc.generate(10, G=16, R=10, F=5, seed=1)
+    with smoc.Context(16, 10, 5) as ctx:
+        ctx.set_regions(prob.sigT, prob.fine_source)
+        off = np.array([0, 1 << 31], np.int64)
+        rc = smoc.lib().smoc_set_segments(ctx._h, 1 << 31, prob.len, prob.region, prob.z, 1,
+                                          off, np.ascontiguousarray(prob.psi_in[:1]).ravel())
+        assert rc == smoc.ERR_ARG
+        assert "2^31" in smoc.lib().smoc_last_error().decode()
