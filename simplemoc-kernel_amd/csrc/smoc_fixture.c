@@ -105,6 +105,48 @@ static int read_arrays(FILE *f, const char *path, uint64_t h0, uint64_t want_has
 
 #define FIX_MAX_COUNT (INT64_C(1) << 40) /* bound on S and T; keeps every byte count in range */
 
+/* the file's size must be exactly what its header implies; checked before
+ * any array is read, so a damaged header cannot make a reader size its
+ * buffers from it */
+static int size_ok(FILE *f, const char *path, uint64_t want)
+{
+    const long here = ftell(f);
+    if (here < 0 || fseek(f, 0, SEEK_END) != 0) {
+        smoc_set_error("%s: cannot determine the file size", path);
+        return SMOC_ERR_ARG;
+    }
+    const long end = ftell(f);
+    if (end < 0 || fseek(f, here, SEEK_SET) != 0) {
+        smoc_set_error("%s: cannot determine the file size", path);
+        return SMOC_ERR_ARG;
+    }
+    if ((uint64_t)end < want) {
+        smoc_set_error("%s is truncated (%ld bytes, header implies %llu)", path, end,
+                       (unsigned long long)want);
+        return SMOC_ERR_ARG;
+    }
+    if ((uint64_t)end > want) {
+        smoc_set_error("%s has trailing bytes (%ld bytes, header implies %llu)", path, end,
+                       (unsigned long long)want);
+        return SMOC_ERR_ARG;
+    }
+    return SMOC_OK;
+}
+
+static uint64_t prb_bytes(const prb_header *h)
+{
+    const uint64_t G = (uint64_t)h->G, R = (uint64_t)h->R, F = (uint64_t)h->F;
+    const uint64_t S = (uint64_t)h->S, T = (uint64_t)h->T;
+    return sizeof(*h) + 4 * (R * G + R * F * G + 3 * S + T * G) + 8 * (T + 1);
+}
+
+static uint64_t res_bytes(const res_header *h)
+{
+    const uint64_t G = (uint64_t)h->G, R = (uint64_t)h->R, F = (uint64_t)h->F;
+    const uint64_t T = (uint64_t)h->T;
+    return sizeof(*h) + 8 * R * F * G + 4 * T * G;
+}
+
 static int shape_ok(int64_t G, int64_t R, int64_t F, int64_t T)
 {
     return G >= 1 && G <= 65536 && R >= 1 && F >= 1 && T >= 0 && T <= FIX_MAX_COUNT &&
@@ -156,7 +198,7 @@ static int read_prb_header(FILE *f, const char *path, prb_header *h)
         smoc_set_error("%s: unsupported version %u or bad shape", path, h->version);
         return SMOC_ERR_ARG;
     }
-    return SMOC_OK;
+    return size_ok(f, path, prb_bytes(h));
 }
 
 int smoc_problem_read_info(const char *path, smoc_problem_info *info)
@@ -248,7 +290,7 @@ static int read_res_header(FILE *f, const char *path, res_header *h)
         smoc_set_error("%s: unsupported version %u or bad shape", path, h->version);
         return SMOC_ERR_ARG;
     }
-    return SMOC_OK;
+    return size_ok(f, path, res_bytes(h));
 }
 
 int smoc_results_read_info(const char *path, smoc_results_info *info)

@@ -143,3 +143,15 @@ def test_driver_writes_the_generated_problem(tmp_path):
     assert r.returncode in (0, 2)  # 2: no device here, after the file was written
     prob = smoc.generate(4000, G=16, R=90, F=5, seed=11, track_len=50)
     same_problem(prob, smoc.load_problem(path))
+
+
+def test_header_claiming_huge_arrays_is_rejected_before_reading(tmp_path):
+    """A header whose counts do not match the file size is refused by the
+    info call itself, before a caller sizes buffers from it."""
+    path = _write_problem(tmp_path)
+    b = bytearray(open(path, "rb").read())
+    b[32:40] = (1 << 39).to_bytes(8, "little")  # T: 2^39 tracks
+    open(path, "wb").write(bytes(b))
+    info = smoc.ProblemInfo()
+    assert smoc.lib().smoc_problem_read_info(path.encode(), info) == smoc.ERR_ARG
+    assert "truncated" in smoc.lib().smoc_last_error().decode()
