@@ -965,7 +965,7 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
         if ((rc = consider(cand)) != SMOC_OK)
             return rc;
     }
-    for (int w : {1, 2, 8}) {
+    for (int w : {1, 2, 4, 8, 16}) {
         smoc_launch cand = best;
         cand.waves_per_block = w;
         if ((rc = consider(cand)) != SMOC_OK)
