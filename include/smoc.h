@@ -170,6 +170,9 @@ int smoc_generate_segments(uint64_t seed, int64_t S, int R, float *len, int32_t 
                            float *z);
 /*   psi_in ~ U(0,1) [T][G]. */
 int smoc_generate_psi(uint64_t seed, int64_t T, int G, float *psi_in);
+/*   scattering ratio ~ cmax * U(0,1) [R][G], 0 <= cmax < 1 (outer iteration;
+ *   no counterpart in the reference). */
+int smoc_generate_scatter(uint64_t seed, int R, int G, float cmax, float *scatter);
 /* Tracks of L consecutive segments (the last one shorter when S % L != 0).
  * Returns the track count via *T; track_off needs ceil(S/L)+1 entries. */
 int smoc_tracks_fixed(int64_t S, int L, int64_t *track_off, int64_t *T);
@@ -269,6 +272,39 @@ int smoc_run(smoc_ctx *ctx, const smoc_launch *launch, smoc_timing *timing);
 /* Copy results to the host: fine_flux [R][F][G] as double, psi_out [T][G].
  * Either pointer may be NULL. */
 int smoc_get_results(smoc_ctx *ctx, double *fine_flux, float *psi_out);
+
+/* ---- outer (source) iteration ----------------------------------------------
+ * No counterpart in the reference kernel: it is the step after the sweep in
+ * the full application the kernel was cut from (README.txt:40-50 describes
+ * the kernel as that application's inner loop).  A fixed-source problem with
+ * isotropic within-group scattering: after each sweep, the angle-averaged
+ * flux of (region r, interval f, group g) is estimated by track length,
+ *   phi = fine_flux / vol,  vol = sum of 0.5 * len over the (r, f) segments,
+ * and the source becomes
+ *   fine_source[r][f][g] = q_ext[r][f][g] + scatter[r][g] * sigT[r][g] * phi
+ * (FP64 arithmetic, stored as FP32 and repacked for the next sweep; rows
+ * without segments keep q_ext).  With scatter < 1 the iteration converges to
+ * the fixed point of source and flux. */
+
+/* scatter [R][G], the scattering ratio sigma_s / sigT in [0, 1); q_ext
+ * [R][F][G], or NULL to take the current fine_source as the external source.
+ * Call after smoc_set_regions (which clears it). */
+int smoc_set_scattering(smoc_ctx *ctx, const float *scatter, const float *q_ext);
+
+/* One source update from the last smoc_run's fine_flux.  change (may be
+ * NULL) receives ||q_new - q_old||_2 / ||q_old||_2. */
+int smoc_source_update(smoc_ctx *ctx, double *change);
+
+/* n_outer iterations of (smoc_run, source update).  timing sums the runs'
+ * timings (a graph launch replays its graph every iteration); update_ms is
+ * the total time of the updates, change the last update's relative change.
+ * Any output pointer may be NULL.  Afterwards fine_flux and psi_out are those
+ * of the last sweep and fine_source the last update's. */
+int smoc_iterate(smoc_ctx *ctx, const smoc_launch *launch, int n_outer, smoc_timing *timing,
+                 double *update_ms, double *change);
+
+/* Copy the current fine_source [R][F][G] (FP32 master) to the host. */
+int smoc_get_source(smoc_ctx *ctx, float *fine_source);
 
 #ifdef __cplusplus
 }

@@ -212,3 +212,32 @@ class Ref:
                                np.ascontiguousarray(psi_in, np.float32).ravel(), psi_out.ravel(),
                                nthreads)
         return sec, psi_out
+
+
+# ---- outer (source) iteration --------------------------------------------------
+# A restatement of the library's source update (numpy, FP64).  The reference
+# kernel has no source update; the update is the fixed-source, isotropic
+# within-group scattering iteration of the application the kernel was cut
+# from, with the track-length estimate of the angle-averaged flux.
+def track_volumes(prob, weight=0.5):
+    """Sum of weight * len over the segments of every (region, interval),
+    FP64, shape [R][F]; the interval is decoded from z with the library's
+    FP32 operations."""
+    F = prob.F
+    zf = (np.asarray(prob.z, np.float32) * np.float32(F)).astype(np.float32)
+    f = np.clip(zf.astype(np.int32), 0, F - 1).astype(np.int64)
+    vol = np.zeros(prob.R * F)
+    np.add.at(vol, np.asarray(prob.region, np.int64) * F + f,
+              weight * np.asarray(prob.len, np.float64))
+    return vol.reshape(prob.R, F)
+
+
+def source_update(fine_flux, vol, sigT, scatter, q_ext):
+    """q_ext + scatter * sigT * fine_flux / vol per (region, interval, group),
+    FP64; (region, interval) rows without segments keep q_ext."""
+    vol = np.asarray(vol, np.float64)
+    phi = np.where(vol[..., None] > 0,
+                   np.asarray(fine_flux, np.float64) / np.where(vol > 0, vol, 1.0)[..., None],
+                   0.0)
+    return (np.asarray(q_ext, np.float64) + np.asarray(scatter, np.float64)[:, None, :] *
+            np.asarray(sigT, np.float64)[:, None, :] * phi)

@@ -110,6 +110,7 @@ struct smoc_ctx {
     DevBuf<float> row_thr;  // [R*F] series threshold per row (EXP_ACCURATE)
     DevBuf<float> flux_init;  // [R*F][Gp] or empty (zeros)
     DevBuf<unsigned char> flux;  // f32 or f64 [R*F][Gp]
+    DevBuf<float> sig_m, src_m;  // FP32 masters on the device: sigT [R][G], source [R][F][G]
     DevBuf<float> table;
     float tab_dx = 0.f, tab_max = 0.f;
     int tab_n = 0;
@@ -129,6 +130,12 @@ struct smoc_ctx {
     int64_t n_runs = 0;
     DevBuf<int64_t> track_run_off, fsr_run_off, fsr_run_idx;
     DevBuf<unsigned char> part;
+
+    // outer (source) iteration
+    DevBuf<float> scatter, q_ext;  // [R][G] scattering ratio, [R][F][G] external source
+    DevBuf<double> vol;            // [R*F] sum of w * len per (region, interval)
+    DevBuf<double> upd;            // per-block partials of the change norm, then 2 sums
+    bool scatter_set = false;
 
     DevBuf<int64_t> queue;  // persistent work counters
     std::vector<int64_t> h_track_off;
@@ -319,6 +326,12 @@ int smoc_destroy(smoc_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     c->tuples.release();
     c->row_thr.release();
+    c->sig_m.release();
+    c->src_m.release();
+    c->scatter.release();
+    c->q_ext.release();
+    c->vol.release();
+    c->upd.release();
     c->flux_init.release();
     c->flux.release();
     c->table.release();
@@ -387,17 +400,18 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
         rc = c->flux.alloc((size_t)nrow * c->Gp * (c->flux_f64() ? 8 : 4) + 256);
     if (rc != SMOC_OK)
         return rc;
-    // FP32 masters are staged on the device, packed (and quantised) there
-    DevBuf<float> dsig, dsrc;
-    rc = dsig.alloc((size_t)nsig);
+    // FP32 masters are kept on the device and packed (and quantised) there;
+    // a source update rewrites the source master and repacks
+    rc = c->sig_m.alloc((size_t)nsig);
     if (rc == SMOC_OK)
-        rc = dsrc.alloc((size_t)nsrc);
+        rc = c->src_m.alloc((size_t)nsrc);
     if (rc != SMOC_OK)
         return rc;
-    HIPCHK(hipMemcpyAsync(dsig.p, sigT, sizeof(float) * nsig, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(dsrc.p, fine_source, sizeof(float) * nsrc, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(c->sig_m.p, sigT, sizeof(float) * nsig, hipMemcpyHostToDevice,
                           c->stream));
-    smoc_pack_args pa{dsig.p, dsrc.p, c->tuples.p, G, c->Gp, R, F};
+    HIPCHK(hipMemcpyAsync(c->src_m.p, fine_source, sizeof(float) * nsrc, hipMemcpyHostToDevice,
+                          c->stream));
+    smoc_pack_args pa{c->sig_m.p, c->src_m.p, c->tuples.p, G, c->Gp, R, F};
     LAUNCHCHK(smoc_launch_pack(&pa, c->p.storage, c->stream));
     LAUNCHCHK(smoc_launch_row_thr(c->tuples.p, c->row_thr.p, nrow, G, c->Gp, c->p.storage,
                                   c->stream));
@@ -414,6 +428,7 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
     c->regions_set = true;
     c->ran = false;
     c->tuned_valid = false;
+    c->scatter_set = false;
     return SMOC_OK;
 }
 
@@ -574,6 +589,21 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
     rc = upload_padded(c->psi_in.p, psi_in, T, G, c->Gp, c->stream);
     if (rc != SMOC_OK)
         return rc;
+    {
+        // track-length volume of each (region, interval): sum of w * len, in
+        // segment order (FP64), for the source update of an outer iteration
+        std::vector<double> v((size_t)c->n_fsr(), 0.0);
+        for (int64_t k = 0; k < S; k++) {
+            int f;
+            float zi;
+            smoc_decode_z(z[k], c->p.fine_axial, &f, &zi);
+            v[(size_t)region[k] * c->p.fine_axial + f] += (double)SMOC_WEIGHT * len[k];
+        }
+        rc = c->vol.alloc(v.size());
+        if (rc != SMOC_OK)
+            return rc;
+        HIPCHK(hipMemcpy(c->vol.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    }
     c->S = S;
     c->T = T;
     c->n_runs = 0;
@@ -1055,6 +1085,151 @@ int smoc_get_results(smoc_ctx *c, double *fine_flux, float *psi_out)
     if (psi_out && c->T > 0)
         HIPCHK(hipMemcpy2D(psi_out, sizeof(float) * G, c->psi_out.p, sizeof(float) * Gp,
                            sizeof(float) * G, (size_t)c->T, hipMemcpyDeviceToHost));
+    return SMOC_OK;
+}
+
+int smoc_set_scattering(smoc_ctx *c, const float *scatter, const float *q_ext)
+{
+    if (!c || !scatter) {
+        smoc_set_error("smoc_set_scattering: NULL argument");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->regions_set) {
+        smoc_set_error("smoc_set_scattering: call smoc_set_regions first");
+        return SMOC_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    const int G = c->p.egroups, R = c->p.regions, F = c->p.fine_axial;
+    const int64_t nsc = (int64_t)R * G, nsrc = (int64_t)R * F * G;
+    for (int64_t i = 0; i < nsc; i++)
+        if (!(scatter[i] >= 0.f && scatter[i] < 1.f)) {
+            smoc_set_error("smoc_set_scattering: scatter[%lld] = %g must be in [0, 1)",
+                           (long long)i, (double)scatter[i]);
+            return SMOC_ERR_ARG;
+        }
+    if (q_ext)
+        for (int64_t i = 0; i < nsrc; i++)
+            if (!std::isfinite(q_ext[i])) {
+                smoc_set_error("smoc_set_scattering: q_ext[%lld] is not finite", (long long)i);
+                return SMOC_ERR_ARG;
+            }
+    int rc = c->scatter.alloc((size_t)nsc);
+    if (rc == SMOC_OK)
+        rc = c->q_ext.alloc((size_t)nsrc);
+    if (rc == SMOC_OK)
+        rc = c->upd.alloc((size_t)(2 * smoc_update_blocks(nsrc) + 2));
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpy(c->scatter.p, scatter, sizeof(float) * nsc, hipMemcpyHostToDevice));
+    if (q_ext)
+        HIPCHK(hipMemcpy(c->q_ext.p, q_ext, sizeof(float) * nsrc, hipMemcpyHostToDevice));
+    else  // the current source is the external one
+        HIPCHK(hipMemcpy(c->q_ext.p, c->src_m.p, sizeof(float) * nsrc, hipMemcpyDeviceToDevice));
+    c->scatter_set = true;
+    return SMOC_OK;
+}
+
+// New source from the last sweep's fine_flux, repacked into the tuples the
+// next sweep reads (same buffers, so a captured sweep graph stays valid).
+static int source_update(smoc_ctx *c, double *change, double *ms)
+{
+    const int G = c->p.egroups, R = c->p.regions, F = c->p.fine_axial;
+    const int64_t nsrc = (int64_t)R * F * G, nblk = smoc_update_blocks(nsrc);
+    smoc_update_args u{};
+    u.flux = c->flux.p + 256;
+    u.vol = c->vol.p;
+    u.sigT = c->sig_m.p;
+    u.scatter = c->scatter.p;
+    u.q_ext = c->q_ext.p;
+    u.source = c->src_m.p;
+    u.partial = c->upd.p;
+    u.sums = c->upd.p + 2 * nblk;
+    u.G = G, u.Gp = c->Gp, u.R = R, u.F = F;
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    LAUNCHCHK(smoc_launch_source_update(&u, c->flux_f64() ? 1 : 0, c->stream));
+    smoc_pack_args pa{c->sig_m.p, c->src_m.p, c->tuples.p, G, c->Gp, R, F};
+    LAUNCHCHK(smoc_launch_pack(&pa, c->p.storage, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    double sums[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(sums, u.sums, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ms) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+        *ms = t;
+    }
+    if (change)
+        *change = sums[1] > 0.0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
+    return SMOC_OK;
+}
+
+int smoc_source_update(smoc_ctx *c, double *change)
+{
+    if (!c) {
+        smoc_set_error("smoc_source_update: NULL context");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->scatter_set || !c->ran) {
+        smoc_set_error("smoc_source_update: needs smoc_set_scattering and a completed smoc_run");
+        return SMOC_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    return source_update(c, change, nullptr);
+}
+
+int smoc_iterate(smoc_ctx *c, const smoc_launch *launch, int n_outer, smoc_timing *timing,
+                 double *update_ms, double *change)
+{
+    if (!c || n_outer < 1) {
+        smoc_set_error("smoc_iterate: NULL context or n_outer < 1");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->scatter_set) {
+        smoc_set_error("smoc_iterate: call smoc_set_scattering first");
+        return SMOC_ERR_STATE;
+    }
+    smoc_timing sum{}, tm{};
+    double upd_ms = 0.0, chg = 0.0;
+    for (int it = 0; it < n_outer; it++) {
+        int rc = smoc_run(c, launch, &tm);
+        if (rc != SMOC_OK)
+            return rc;
+        double ms = 0.0;
+        rc = source_update(c, &chg, &ms);
+        if (rc != SMOC_OK)
+            return rc;
+        upd_ms += ms;
+        sum.total_ms += tm.total_ms;
+        sum.reset_ms += tm.reset_ms;
+        sum.xfer_ms += tm.xfer_ms;
+        sum.sweep_ms += tm.sweep_ms;
+        sum.reduce_ms += tm.reduce_ms;
+        sum.segments += tm.segments;
+        sum.work_items = tm.work_items;
+        sum.used = tm.used;
+    }
+    if (timing)
+        *timing = sum;
+    if (update_ms)
+        *update_ms = upd_ms;
+    if (change)
+        *change = chg;
+    return SMOC_OK;
+}
+
+int smoc_get_source(smoc_ctx *c, float *fine_source)
+{
+    if (!c || !fine_source) {
+        smoc_set_error("smoc_get_source: NULL argument");
+        return SMOC_ERR_ARG;
+    }
+    if (!c->regions_set) {
+        smoc_set_error("smoc_get_source: call smoc_set_regions first");
+        return SMOC_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(c->p.device));
+    const int64_t nsrc = (int64_t)c->p.regions * c->p.fine_axial * c->p.egroups;
+    HIPCHK(hipMemcpy(fine_source, c->src_m.p, sizeof(float) * nsrc, hipMemcpyDeviceToHost));
     return SMOC_OK;
 }
 

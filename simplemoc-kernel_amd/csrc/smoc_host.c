@@ -155,7 +155,16 @@ static inline int32_t below(uint64_t x, uint32_t n)
     return (int32_t)(((x >> 32) * (uint64_t)n) >> 32);
 }
 
-enum { ST_SIGT = 1, ST_SOURCE = 2, ST_LEN = 3, ST_REGION = 4, ST_Z = 5, ST_PSI = 6, ST_TRACK = 7 };
+enum {
+    ST_SIGT = 1,
+    ST_SOURCE = 2,
+    ST_LEN = 3,
+    ST_REGION = 4,
+    ST_Z = 5,
+    ST_PSI = 6,
+    ST_TRACK = 7,
+    ST_SCATTER = 8
+};
 
 int smoc_generate_regions(uint64_t seed, int G, int R, int F, float *sigT, float *fine_source)
 {
@@ -202,6 +211,20 @@ int smoc_generate_psi(uint64_t seed, int64_t T, int G, float *psi_in)
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++)
         psi_in[i] = u01(draw(kp, (uint64_t)i));
+    return SMOC_OK;
+}
+
+int smoc_generate_scatter(uint64_t seed, int R, int G, float cmax, float *scatter)
+{
+    if (R <= 0 || G <= 0 || !scatter || !(cmax >= 0.f && cmax < 1.f)) {
+        smoc_set_error("smoc_generate_scatter: bad argument (cmax must be in [0, 1))");
+        return SMOC_ERR_ARG;
+    }
+    const uint64_t ks = stream_key(seed, ST_SCATTER);
+    const int64_t n = (int64_t)R * G;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        scatter[i] = cmax * u01(draw(ks, (uint64_t)i));
     return SMOC_OK;
 }
 

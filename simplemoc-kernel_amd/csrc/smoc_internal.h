@@ -87,6 +87,19 @@ typedef struct {
     int single;                  /* partials are single FP32 tallies */
 } smoc_reduce_args;
 
+/* Source update of an outer iteration (see k_source_update). */
+typedef struct {
+    const void *flux;       /* fine_flux [R*F][Gp], f32 or f64 */
+    const double *vol;      /* [R*F] sum of w * len per (region, interval) */
+    const float *sigT;      /* [R][G] FP32 master */
+    const float *scatter;   /* [R][G] c = sigma_s / sigT */
+    const float *q_ext;     /* [R][F][G] external source */
+    float *source;          /* [R][F][G] in: current master, out: updated */
+    double *partial;        /* [blocks][2] per-block sums */
+    double *sums;           /* [2] sum (q_new - q_old)^2, sum q_old^2 */
+    int G, Gp, R, F;
+} smoc_update_args;
+
 /* kernel launchers (smoc_kernels.hip); return 0 or a hipError_t value */
 int smoc_launch_pack(const smoc_pack_args *a, int storage, void *stream);
 int smoc_launch_row_thr(const void *tuples, float *thr, int64_t nrow, int G, int Gp,
@@ -98,6 +111,8 @@ int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_
                        void *stream);
 int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream);
 int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
+int64_t smoc_update_blocks(int64_t n);
+int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *stream);
 int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);

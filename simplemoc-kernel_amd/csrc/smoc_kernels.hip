@@ -898,6 +898,81 @@ __global__ void __launch_bounds__(256) k_row_thr(const void *tuples, float *thr,
         thr[row] = (1.f - 0x1p-20f) / m;
 }
 
+// ---- source update of an outer (source) iteration -------------------------------
+// After a sweep, the track-length estimate of the angle-averaged flux of
+// (region r, interval f, group g) is phi = fine_flux / vol, vol = sum of
+// w * len over the segments in (r, f); isotropic within-group scattering with
+// ratio c = sigma_s / sigT gives the new source
+//   q[r][f][g] = q_ext[r][f][g] + c[r][g] * sigT[r][g] * phi[r][f][g]
+// (FP64 arithmetic, stored as the FP32 master).  Rows without segments keep
+// q_ext.  Each block also writes its partial sums of (q_new - q_old)^2 and
+// q_old^2 in a fixed order, for a deterministic change norm.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_source_update(smoc_update_args a)
+{
+    __shared__ double s_d[256], s_o[256];
+    const int64_t n = (int64_t)a.R * a.F * a.G;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0, o2 = 0.0;
+    if (i < n) {
+        const int G = a.G;
+        const int64_t row = i / G;
+        const int g = (int)(i - row * G);
+        const int64_t r = row / a.F;
+        const double v = a.vol[row];
+        const double phi = v > 0.0 ? (double)static_cast<const FT *>(a.flux)[row * a.Gp + g] / v
+                                   : 0.0;
+        const double q = (double)a.q_ext[i] +
+                         (double)a.scatter[r * G + g] * (double)a.sigT[r * G + g] * phi;
+        const float qf = (float)q;
+        const float old = a.source[i];
+        a.source[i] = qf;
+        d2 = ((double)qf - old) * ((double)qf - old);
+        o2 = (double)old * old;
+    }
+    s_d[threadIdx.x] = d2;
+    s_o[threadIdx.x] = o2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_d[threadIdx.x] += s_d[threadIdx.x + o];
+            s_o[threadIdx.x] += s_o[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.partial[2 * blockIdx.x] = s_d[0];
+        a.partial[2 * blockIdx.x + 1] = s_o[0];
+    }
+}
+
+// Sum of the per-block partials in a fixed order (one block of 256: thread t
+// sums blocks t, t + 256, ... in turn, then a tree over the threads).
+__global__ void __launch_bounds__(256) k_sum_partials(const double *partial, int64_t nblk,
+                                                      double *out)
+{
+    __shared__ double s_d[256], s_o[256];
+    double d = 0.0, o = 0.0;
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) {
+        d += partial[2 * b];
+        o += partial[2 * b + 1];
+    }
+    s_d[threadIdx.x] = d;
+    s_o[threadIdx.x] = o;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            s_d[threadIdx.x] += s_d[threadIdx.x + h];
+            s_o[threadIdx.x] += s_o[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_d[0];
+        out[1] = s_o[0];
+    }
+}
+
 // ---- segment stream encoding (once per smoc_set_segments) -----------------------
 // In place: zin_io holds z on entry and zin on exit; code_io holds the region
 // id on entry and (region * F + f) << 2 | fit kind on exit.
@@ -1157,6 +1232,23 @@ extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int gri
         return 0;
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    return (int)hipGetLastError();
+}
+
+extern "C" int64_t smoc_update_blocks(int64_t n) { return n > 0 ? blocks_for(n, 256) : 0; }
+
+extern "C" int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *stream)
+{
+    const int64_t n = (int64_t)a->R * a->F * a->G;
+    const int64_t nb = smoc_update_blocks(n);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (nb > 0) {
+        if (flux_f64)
+            hipLaunchKernelGGL(k_source_update<double>, dim3((unsigned)nb), dim3(256), 0, s, *a);
+        else
+            hipLaunchKernelGGL(k_source_update<float>, dim3((unsigned)nb), dim3(256), 0, s, *a);
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, a->partial, nb, a->sums);
     return (int)hipGetLastError();
 }
 

@@ -30,6 +30,8 @@ typedef struct {
     int p_given;
     const char *save_problem, *load_problem, *save_results, *check_results;
     double tol;
+    int outer;
+    double scatter;
     smoc_launch launch;
 } options;
 
@@ -81,6 +83,8 @@ static void print_CLI_error(void)
     printf("  --save-problem F / --load-problem F   write / read the inputs (fixture file)\n");
     printf("  --save-results F / --check-results F  write results / compare with a results\n");
     printf("                         file (exit 3 when an error exceeds --tol, default 1e-5)\n");
+    printf("  --outer N --scatter C  then N outer (source) iterations with scattering ratios\n");
+    printf("                         C * U(0,1) per region and group (default C 0.5)\n");
     printf("See README for the default run values\n");
     exit(1);
 }
@@ -154,6 +158,10 @@ static void read_CLI(int argc, char *argv[], options *o)
             o->check_results = NEXT();
         else if (strcmp(arg, "--tol") == 0)
             o->tol = atof(NEXT());
+        else if (strcmp(arg, "--outer") == 0)
+            o->outer = atoi(NEXT());
+        else if (strcmp(arg, "--scatter") == 0)
+            o->scatter = atof(NEXT());
         else
             print_CLI_error();
 #undef NEXT
@@ -182,7 +190,12 @@ int main(int argc, char *argv[])
     o.storage = SMOC_STORE_F32;
     o.tally = SMOC_TALLY_ATOMIC_F32;
     o.tol = 1e-5;
+    o.scatter = 0.5;
     read_CLI(argc, argv, &o);
+    if (o.outer < 0 || !(o.scatter >= 0.0 && o.scatter < 1.0)) {
+        fprintf(stderr, "--outer must be >= 0 and --scatter in [0, 1)\n");
+        return 1;
+    }
     smoc_set_host_threads(o.nthreads);
     smoc_derive_regions(&o.in);
     int R = o.regions_override > 0 ? o.regions_override : o.in.source_3D_regions;
@@ -355,6 +368,31 @@ int main(int argc, char *argv[])
     printf("%-25s%.17g\n", "fine_flux checksum:", ck_flux);
     printf("%-25s%.17g\n", "state_flux checksum:", ck_psi);
     border_print();
+    if (o.outer > 0) {
+        float *sc = (float *)malloc(sizeof(float) * (size_t)R * G);
+        if (!sc)
+            die("host allocation", SMOC_ERR_NOMEM);
+        if ((rc = smoc_generate_scatter(o.seed, R, G, (float)o.scatter, sc)))
+            die("smoc_generate_scatter", rc);
+        if ((rc = smoc_set_scattering(ctx, sc, NULL)))
+            die("smoc_set_scattering", rc);
+        free(sc);
+        smoc_timing ot;
+        double upd_ms = 0.0, chg = 0.0;
+        if ((rc = smoc_iterate(ctx, &o.launch, o.outer, &ot, &upd_ms, &chg)))
+            die("smoc_iterate", rc);
+        if ((rc = smoc_get_results(ctx, flux, psi_out)))
+            die("smoc_get_results", rc);
+        center_print("OUTER ITERATION", 80);
+        border_print();
+        printf("%-25s%d (scattering ratio up to %.2f)\n", "Outer iterations:", o.outer, o.scatter);
+        printf("%-25s%.3f ms sweep, %.3f ms source update\n", "Per iteration:",
+               ot.total_ms / o.outer, upd_ms / o.outer);
+        printf("%-25s%.3e\n", "Last source change:", chg);
+        printf("%-25s%.17g\n", "fine_flux checksum:", smoc_checksum_f64(flux, (int64_t)R * F * G));
+        printf("%-25s%.17g\n", "state_flux checksum:", smoc_checksum_f32(psi_out, T * G));
+        border_print();
+    }
     int status = 0;
     if (o.save_results) {
         smoc_results_info ri = {G, R, F, T, o.storage, o.exp_mode, o.tally, 0.0, 0.0};

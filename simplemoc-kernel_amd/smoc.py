@@ -127,6 +127,12 @@ SIGNATURES = {
     "smoc_set_segments": (_i, [_P, _i64, _F32, _I32, _F32, _i64, _I64, _F32]),
     "smoc_run": (_i, [_P, ctypes.POINTER(Launch), ctypes.POINTER(Timing)]),
     "smoc_get_results": (_i, [_P, _P, _P]),
+    "smoc_set_scattering": (_i, [_P, _F32, _P]),
+    "smoc_generate_scatter": (_i, [_u64, _i, _i, _f, _F32]),
+    "smoc_source_update": (_i, [_P, ctypes.POINTER(_d)]),
+    "smoc_iterate": (_i, [_P, ctypes.POINTER(Launch), _i, ctypes.POINTER(Timing),
+                          ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "smoc_get_source": (_i, [_P, _F32]),
 }
 
 _lib = None
@@ -327,6 +333,33 @@ class Context:
                "smoc_get_results")
         return flux, psi
 
+    # outer (source) iteration
+    def set_scattering(self, scatter, q_ext=None):
+        scatter = np.ascontiguousarray(scatter, np.float32).ravel()
+        assert scatter.size == self.R * self.G
+        q = None
+        if q_ext is not None:
+            q = np.ascontiguousarray(q_ext, np.float32).ravel()
+            assert q.size == self.R * self.F * self.G
+        _check(lib().smoc_set_scattering(self._h, scatter, q.ctypes.data if q is not None else None),
+               "smoc_set_scattering")
+
+    def source_update(self) -> float:
+        chg = _d()
+        _check(lib().smoc_source_update(self._h, ctypes.byref(chg)), "smoc_source_update")
+        return chg.value
+
+    def iterate(self, n_outer: int, **launch):
+        ln, tm, ums, chg = Launch(**launch), Timing(), _d(), _d()
+        _check(lib().smoc_iterate(self._h, ctypes.byref(ln), n_outer, ctypes.byref(tm),
+                                  ctypes.byref(ums), ctypes.byref(chg)), "smoc_iterate")
+        return tm, ums.value, chg.value
+
+    def source(self):
+        out = np.empty((self.R, self.F, self.G), np.float32)
+        _check(lib().smoc_get_source(self._h, out.ravel()), "smoc_get_source")
+        return out
+
 
 def checksums(flux: np.ndarray, psi: np.ndarray):
     """Fixed-order FP64 checksums of fine_flux and state_flux (libsmoc's)."""
@@ -395,3 +428,10 @@ def compare_results(path: str, flux, psi):
                                       *[ctypes.byref(o) for o in out]), "smoc_results_compare")
     return dict(zip(("fine_checksum_rel", "state_checksum_rel", "fine_normwise",
                      "state_normwise"), (o.value for o in out)))
+
+
+def generate_scatter(R: int, G: int, cmax: float, seed: int = 42) -> np.ndarray:
+    """Scattering ratios cmax * U(0,1) [R][G] from the library's generator."""
+    out = np.empty((R, G), np.float32)
+    _check(lib().smoc_generate_scatter(seed, R, G, cmax, out.ravel()), "smoc_generate_scatter")
+    return out

@@ -181,3 +181,14 @@ def test_generator_rejects_bad_arguments():
     t = ctypes.c_int64()
     assert L.smoc_tracks_fixed(10, 0, off, ctypes.byref(t)) == smoc.ERR_ARG
     assert b"smoc_tracks_fixed" in L.smoc_last_error()
+
+
+def test_generate_scatter():
+    a = smoc.generate_scatter(40, 16, 0.7, seed=9)
+    b = smoc.generate_scatter(40, 16, 0.7, seed=9)
+    c = smoc.generate_scatter(40, 16, 0.7, seed=10)
+    assert a.shape == (40, 16) and np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.min() >= 0 and a.max() < 0.7 and abs(a.mean() - 0.35) < 0.03
+    for bad in (1.0, -0.1):
+        with pytest.raises(smoc.SmocError):
+            smoc.generate_scatter(4, 4, bad)
