@@ -273,6 +273,30 @@ int smoc_run(smoc_ctx *ctx, const smoc_launch *launch, smoc_timing *timing);
  * Either pointer may be NULL. */
 int smoc_get_results(smoc_ctx *ctx, double *fine_flux, float *psi_out);
 
+/* ---- track producer: ray tracing through a pin-cell lattice ----------------
+ * Replaces the reference's random region and interval draws
+ * (src/cpu/kernel.c:47-50) by the step a transport code performs before the
+ * sweep.  Geometry: nx x ny square cells of side pitch, each with a centred
+ * pin of radius pin_radius, extruded over height and cut into nz axial
+ * layers.  Region ids ((iy * nx + ix) * 2 + outside) * nz + layer, so the
+ * context needs R = 2 nx ny nz regions.  Track t starts from a point drawn
+ * uniformly in the box with an isotropic direction and runs between the two
+ * box faces it meets; its segments are the pieces between cell faces, pin
+ * surfaces and layer planes; z is a segment midpoint's position in its
+ * layer, in [0, 1).  Traced on the GPU in FP64. */
+typedef struct {
+    int nx, ny, nz;
+    double pitch, pin_radius, height;
+} smoc_geometry;
+
+/* Segments per track: track_off [T+1] (exclusive prefix sums) and S. */
+int smoc_trace_count(int device, const smoc_geometry *geometry, uint64_t seed, int64_t T,
+                     int64_t *track_off, int64_t *S);
+/* The segments themselves, len / region / z [S], for the track_off that
+ * smoc_trace_count returned with the same geometry, seed and T. */
+int smoc_trace_segments(int device, const smoc_geometry *geometry, uint64_t seed, int64_t T,
+                        const int64_t *track_off, float *len, int32_t *region, float *z);
+
 /* ---- outer (source) iteration ----------------------------------------------
  * No counterpart in the reference kernel: it is the step after the sweep in
  * the full application the kernel was cut from (README.txt:40-50 describes

@@ -241,3 +241,51 @@ def source_update(fine_flux, vol, sigT, scatter, q_ext):
                    0.0)
     return (np.asarray(q_ext, np.float64) + np.asarray(scatter, np.float64)[:, None, :] *
             np.asarray(sigT, np.float64)[:, None, :] * phi)
+
+
+# ---- track producer restatement --------------------------------------------------
+class Geometry(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("pitch", ctypes.c_double), ("pin_radius", ctypes.c_double),
+                ("height", ctypes.c_double)]
+
+
+class Ray(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ex", "ey", "ez", "dx", "dy", "dz", "L")]
+
+
+def _trace_lib():
+    L = lib()
+    if not getattr(L, "_trace_bound", False):
+        L.oracle_trace_ray.argtypes = [ctypes.POINTER(Geometry), ctypes.c_uint64, ctypes.c_int64,
+                                       ctypes.POINTER(Ray)]
+        L.oracle_trace_ray.restype = None
+        L.oracle_trace_track.argtypes = [ctypes.POINTER(Geometry), ctypes.c_uint64,
+                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_trace_track.restype = ctypes.c_int64
+        L._trace_bound = True
+    return L
+
+
+def trace_ray(g: Geometry, seed: int, t: int) -> Ray:
+    r = Ray()
+    _trace_lib().oracle_trace_ray(ctypes.byref(g), seed, t, ctypes.byref(r))
+    return r
+
+
+def trace(g: Geometry, seed: int, T: int):
+    """All T tracks: (track_off [T+1], len [S], region [S], z [S])."""
+    L = _trace_lib()
+    counts = [L.oracle_trace_track(ctypes.byref(g), seed, t, 0, None, None, None)
+              for t in range(T)]
+    off = np.zeros(T + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    S = int(off[-1])
+    ln, rg, zz = np.empty(S, np.float32), np.empty(S, np.int32), np.empty(S, np.float32)
+    for t in range(T):
+        b, n = int(off[t]), int(counts[t])
+        if n:
+            L.oracle_trace_track(ctypes.byref(g), seed, t, n, ln[b:].ctypes.data,
+                                 rg[b:].ctypes.data, zz[b:].ctypes.data)
+    return off, ln, rg, zz

@@ -357,3 +357,151 @@ double oracle_sum_f32(const float *x, int64_t n)
         s += (double)x[i];
     return s;
 }
+
+
+/* ---- track producer restatement ------------------------------------------- */
+/* counter-based generator of the library: draw(key, i) = mix(key + (i+1) *
+ * golden), key = mix(seed * golden + stream); the tracer's stream id is 9 and
+ * track t uses draw indices t * 64 .. t * 64 + 63. */
+static uint64_t tr_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+#define TR_GOLDEN 0x9E3779B97F4A7C15ULL
+static double tr_u53(uint64_t key, uint64_t i)
+{
+    return (double)(tr_mix(key + (i + 1) * TR_GOLDEN) >> 11) * 0x1p-53;
+}
+static double tr_face(double p, double d, double hi)
+{
+    if (d > 0.0)
+        return (hi - p) / d;
+    if (d < 0.0)
+        return (0.0 - p) / d;
+    return 1e300;
+}
+
+void oracle_trace_ray(const oracle_geometry *g, uint64_t seed, int64_t t, oracle_ray *r)
+{
+    const uint64_t key = tr_mix(seed * TR_GOLDEN + 9);
+    const double X = g->nx * g->pitch, Y = g->ny * g->pitch, Z = g->height;
+    const uint64_t b = (uint64_t)t * 64;
+    const double px = X * tr_u53(key, b), py = Y * tr_u53(key, b + 1), pz = Z * tr_u53(key, b + 2);
+    double dx = 0.0, dy = 0.0, dz = 1.0;
+    /* isotropic direction, Marsaglia's disc method (up to 30 attempts) */
+    for (int k = 0; k < 30; k++) {
+        const double a = 2.0 * tr_u53(key, b + 3 + 2 * k) - 1.0;
+        const double c = 2.0 * tr_u53(key, b + 4 + 2 * k) - 1.0;
+        const double s = a * a + c * c;
+        if (s > 0.0 && s < 1.0) {
+            const double w = 2.0 * sqrt(1.0 - s);
+            dx = a * w;
+            dy = c * w;
+            dz = 1.0 - 2.0 * s;
+            break;
+        }
+    }
+    const double back = fmin(fmin(tr_face(px, -dx, X), tr_face(py, -dy, Y)), tr_face(pz, -dz, Z));
+    const double fwd = fmin(fmin(tr_face(px, dx, X), tr_face(py, dy, Y)), tr_face(pz, dz, Z));
+    r->ex = px - back * dx;
+    r->ey = py - back * dy;
+    r->ez = pz - back * dz;
+    r->dx = dx;
+    r->dy = dy;
+    r->dz = dz;
+    r->L = back + fwd;
+}
+
+static int tr_clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+int64_t oracle_trace_track(const oracle_geometry *g, uint64_t seed, int64_t t, int64_t cap,
+                           float *len, int32_t *region, float *z)
+{
+    oracle_ray r;
+    oracle_trace_ray(g, seed, t, &r);
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const double p = g->pitch, rr = g->pin_radius * g->pin_radius, dzl = g->height / nz;
+    int ix = tr_clampi((int)floor((r.ex + 1e-9 * r.dx) / p), nx - 1);
+    int iy = tr_clampi((int)floor((r.ey + 1e-9 * r.dy) / p), ny - 1);
+    const int sx = r.dx > 0.0 ? 1 : -1, sy = r.dy > 0.0 ? 1 : -1;
+    int64_t n = 0;
+    double tc = 0.0;
+    while (tc < r.L && ix >= 0 && ix < nx && iy >= 0 && iy < ny) {
+        /* next cell faces, as parameters from the entry point */
+        const double tx = r.dx != 0.0 ? ((ix + (sx > 0 ? 1 : 0)) * p - r.ex) / r.dx : 1e300;
+        const double ty = r.dy != 0.0 ? ((iy + (sy > 0 ? 1 : 0)) * p - r.ey) / r.dy : 1e300;
+        double tend = fmin(fmin(tx, ty), r.L);
+        if (tend < tc)
+            tend = tc;
+        /* the cell's pin circle */
+        const double cx = (ix + 0.5) * p, cy = (iy + 0.5) * p;
+        const double ox = r.ex - cx, oy = r.ey - cy;
+        const double A = r.dx * r.dx + r.dy * r.dy;
+        const double B = ox * r.dx + oy * r.dy;
+        const double C = ox * ox + oy * oy - rr;
+        double cut[4];
+        int nc = 0;
+        cut[nc++] = tc;
+        if (A > 0.0) {
+            const double disc = B * B - A * C;
+            if (disc > 0.0) {
+                const double sq = sqrt(disc);
+                const double s1 = (-B - sq) / A, s2 = (-B + sq) / A;
+                if (s1 > tc && s1 < tend)
+                    cut[nc++] = s1;
+                if (s2 > tc && s2 < tend)
+                    cut[nc++] = s2;
+            }
+        }
+        cut[nc++] = tend;
+        for (int k = 0; k + 1 < nc; k++) {
+            const double a = cut[k], b = cut[k + 1];
+            if (!(b > a))
+                continue;
+            const double sm = 0.5 * (a + b);
+            const double qx = r.ex + sm * r.dx - cx, qy = r.ey + sm * r.dy - cy;
+            const int outside = qx * qx + qy * qy < rr ? 0 : 1;
+            const int64_t r2 = ((int64_t)iy * nx + ix) * 2 + outside;
+            double s0 = a;
+            while (s0 < b) {
+                const double zs = r.ez + s0 * r.dz;
+                int layer;
+                double s1;
+                if (r.dz > 0.0) {
+                    layer = tr_clampi((int)floor(zs / dzl), nz - 1);
+                    s1 = ((layer + 1) * dzl - r.ez) / r.dz;
+                } else if (r.dz < 0.0) {
+                    layer = tr_clampi((int)ceil(zs / dzl) - 1, nz - 1);
+                    s1 = (layer * dzl - r.ez) / r.dz;
+                } else {
+                    layer = tr_clampi((int)floor(zs / dzl), nz - 1);
+                    s1 = b;
+                }
+                if (!(s1 > s0))
+                    s1 = s0 + 1e-12;
+                if (!(s1 > s0) || s1 > b)
+                    s1 = b;
+                const double zm = r.ez + 0.5 * (s0 + s1) * r.dz;
+                double zf = (zm - layer * dzl) / dzl;
+                zf = zf < 0.0 ? 0.0 : (zf >= 1.0 ? 0x1.fffffep-1 : zf);
+                if (n < cap) {
+                    len[n] = (float)(s1 - s0);
+                    region[n] = (int32_t)(r2 * nz + layer);
+                    z[n] = (float)zf;
+                }
+                n++;
+                s0 = s1;
+            }
+        }
+        tc = tend;
+        if (tend >= r.L)
+            break;
+        if (tx <= ty)
+            ix += sx;
+        if (ty <= tx)
+            iy += sy;
+    }
+    return n;
+}

@@ -101,6 +101,32 @@ double oracle_baseline_f32(const oracle_problem *p, float *fine_flux, float *psi
 double oracle_sum_f64(const double *x, int64_t n);
 double oracle_sum_f32(const float *x, int64_t n);
 
+/* ---- track producer (ray tracing through a pin-cell lattice) --------------
+ * The reference draws regions at random (src/cpu/kernel.c:47-50); the
+ * library's producer ray-traces tracks through an nx x ny lattice of pin
+ * cells with nz axial layers (see the product's smoc_geometry).  This is a
+ * restatement of that tracer in C, FP64 without contraction: the library's
+ * GPU tracer must give the same segments bit for bit.  Pins (tests): the
+ * segment lengths of a track add up to its chord, every segment's region is
+ * that of its midpoint by brute force, z lies in [0, 1) and matches the
+ * midpoint's height, and the share of track length inside pins approaches
+ * the pin volume fraction pi r^2 / pitch^2. */
+typedef struct {
+    int nx, ny, nz;
+    double pitch, pin_radius, height;
+} oracle_geometry;
+
+typedef struct {
+    double ex, ey, ez, dx, dy, dz, L;
+} oracle_ray;
+
+/* Track t's chord (entry point, unit direction, length). */
+void oracle_trace_ray(const oracle_geometry *g, uint64_t seed, int64_t t, oracle_ray *r);
+/* Segments of track t: writes up to cap of them (len, region, z may be NULL
+ * when cap is 0) and returns how many the track has. */
+int64_t oracle_trace_track(const oracle_geometry *g, uint64_t seed, int64_t t, int64_t cap,
+                           float *len, int32_t *region, float *z);
+
 #ifdef __cplusplus
 }
 #endif

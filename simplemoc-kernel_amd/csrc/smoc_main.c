@@ -32,6 +32,8 @@ typedef struct {
     double tol;
     int outer;
     double scatter;
+    int64_t trace_tracks;  /* > 0: ray-traced tracks instead of generated segments */
+    smoc_geometry geo;
     smoc_launch launch;
 } options;
 
@@ -83,6 +85,10 @@ static void print_CLI_error(void)
     printf("  --save-problem F / --load-problem F   write / read the inputs (fixture file)\n");
     printf("  --save-results F / --check-results F  write results / compare with a results\n");
     printf("                         file (exit 3 when an error exceeds --tol, default 1e-5)\n");
+    printf("  --trace T              T ray-traced tracks through a pin-cell lattice (regions\n");
+    printf("                         and segments from the geometry; -s and --regions unused)\n");
+    printf("  --geometry nx,ny,nz[,pitch,radius,height]  lattice for --trace (default\n");
+    printf("                         50,50,27,1.26,0.54,366: 5000 2D regions, 27 layers)\n");
     printf("  --outer N --scatter C  then N outer (source) iterations with scattering ratios\n");
     printf("                         C * U(0,1) per region and group (default C 0.5)\n");
     printf("See README for the default run values\n");
@@ -158,6 +164,14 @@ static void read_CLI(int argc, char *argv[], options *o)
             o->check_results = NEXT();
         else if (strcmp(arg, "--tol") == 0)
             o->tol = atof(NEXT());
+        else if (strcmp(arg, "--trace") == 0)
+            o->trace_tracks = atoll(NEXT());
+        else if (strcmp(arg, "--geometry") == 0) {
+            const char *gs = NEXT();
+            if (sscanf(gs, "%d,%d,%d,%lf,%lf,%lf", &o->geo.nx, &o->geo.ny, &o->geo.nz,
+                       &o->geo.pitch, &o->geo.pin_radius, &o->geo.height) < 3)
+                print_CLI_error();
+        }
         else if (strcmp(arg, "--outer") == 0)
             o->outer = atoi(NEXT());
         else if (strcmp(arg, "--scatter") == 0)
@@ -191,6 +205,8 @@ int main(int argc, char *argv[])
     o.tally = SMOC_TALLY_ATOMIC_F32;
     o.tol = 1e-5;
     o.scatter = 0.5;
+    o.geo.nx = 50, o.geo.ny = 50, o.geo.nz = 27;
+    o.geo.pitch = 1.26, o.geo.pin_radius = 0.54, o.geo.height = 366.0;
     read_CLI(argc, argv, &o);
     if (o.outer < 0 || !(o.scatter >= 0.0 && o.scatter < 1.0)) {
         fprintf(stderr, "--outer must be >= 0 and --scatter in [0, 1)\n");
@@ -217,7 +233,17 @@ int main(int argc, char *argv[])
 
     int64_t S = o.in.segments, T = 0;
     int64_t *track_off = NULL;
-    if (o.load_problem) {
+    if (o.trace_tracks > 0 && !o.load_problem) {
+        R = 2 * o.geo.nx * o.geo.ny * o.geo.nz;
+        T = o.trace_tracks;
+        track_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(T + 1));
+        if (!track_off)
+            die("track allocation", SMOC_ERR_NOMEM);
+        int rc = smoc_trace_count(o.device, &o.geo, o.seed, T, track_off, &S);
+        if (rc)
+            die("smoc_trace_count", rc);
+        o.in.segments = S;
+    } else if (o.load_problem) {
         T = pinfo.T;
         track_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(T + 1));
         if (!track_off)
@@ -277,6 +303,10 @@ int main(int argc, char *argv[])
     printf("%-25s%s\n", "Tally:",
            (const char *[]){"atomic-f32", "atomic-f64", "det-f64", "det-kahan", "fixed64"}[o.tally]);
     printf("%-25s%s\n", "Stream order:", o.sorted ? "sorted by (region, interval)" : "track");
+    if (o.trace_tracks > 0)
+        printf("%-25s%d x %d pin cells (pitch %.3g, pin radius %.3g), %d layers over %.4g\n",
+               "Traced geometry:", o.geo.nx, o.geo.ny, o.geo.pitch, o.geo.pin_radius, o.geo.nz,
+               o.geo.height);
     border_print();
 
     center_print("INITIALIZATION", 80);
@@ -300,8 +330,13 @@ int main(int argc, char *argv[])
     } else {
         if ((rc = smoc_generate_regions(o.seed, G, R, F, sigT, src)))
             die("smoc_generate_regions", rc);
-        if ((rc = smoc_generate_segments(o.seed, S, R, len, region, z)))
+        if (o.trace_tracks > 0) {
+            if ((rc = smoc_trace_segments(o.device, &o.geo, o.seed, T, track_off, len, region,
+                                          z)))
+                die("smoc_trace_segments", rc);
+        } else if ((rc = smoc_generate_segments(o.seed, S, R, len, region, z))) {
             die("smoc_generate_segments", rc);
+        }
         if ((rc = smoc_generate_psi(o.seed, T, G, psi_in)))
             die("smoc_generate_psi", rc);
         if (o.sorted && (rc = smoc_sort_segments(S, R, F, len, region, z)))

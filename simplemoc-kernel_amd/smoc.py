@@ -82,6 +82,17 @@ class ResultsInfo(ctypes.Structure):
                 ("state_flux_checksum", ctypes.c_double)]
 
 
+class Geometry(ctypes.Structure):
+    """Pin-cell lattice of the track producer (smoc_geometry)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("pitch", ctypes.c_double), ("pin_radius", ctypes.c_double),
+                ("height", ctypes.c_double)]
+
+    @property
+    def regions(self) -> int:
+        return 2 * self.nx * self.ny * self.nz
+
+
 _F32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _F64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _I32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -127,6 +138,9 @@ SIGNATURES = {
     "smoc_set_segments": (_i, [_P, _i64, _F32, _I32, _F32, _i64, _I64, _F32]),
     "smoc_run": (_i, [_P, ctypes.POINTER(Launch), ctypes.POINTER(Timing)]),
     "smoc_get_results": (_i, [_P, _P, _P]),
+    "smoc_trace_count": (_i, [_i, ctypes.POINTER(Geometry), _u64, _i64, _I64,
+                              ctypes.POINTER(_i64)]),
+    "smoc_trace_segments": (_i, [_i, ctypes.POINTER(Geometry), _u64, _i64, _I64, _P, _P, _P]),
     "smoc_set_scattering": (_i, [_P, _F32, _P]),
     "smoc_generate_scatter": (_i, [_u64, _i, _i, _f, _F32]),
     "smoc_source_update": (_i, [_P, ctypes.POINTER(_d)]),
@@ -435,3 +449,31 @@ def generate_scatter(R: int, G: int, cmax: float, seed: int = 42) -> np.ndarray:
     out = np.empty((R, G), np.float32)
     _check(lib().smoc_generate_scatter(seed, R, G, cmax, out.ravel()), "smoc_generate_scatter")
     return out
+
+
+def trace(geometry: Geometry, T: int, seed: int = 42, device: int = 0):
+    """Ray-trace T tracks through the pin-cell lattice on the GPU:
+    (track_off [T+1], len [S], region [S], z [S])."""
+    off = np.empty(T + 1, np.int64)
+    S = _i64()
+    _check(lib().smoc_trace_count(device, ctypes.byref(geometry), seed, T, off,
+                                  ctypes.byref(S)), "smoc_trace_count")
+    n = S.value
+    ln, rg, zz = np.empty(n, np.float32), np.empty(n, np.int32), np.empty(n, np.float32)
+    _check(lib().smoc_trace_segments(device, ctypes.byref(geometry), seed, T, off, _ptr(ln),
+                                     _ptr(rg), _ptr(zz)), "smoc_trace_segments")
+    return off, ln, rg, zz
+
+
+def traced_problem(geometry: Geometry, T: int, G: int = 100, F: int = 5, seed: int = 42,
+                   device: int = 0) -> Problem:
+    """A problem whose segment stream comes from the tracer; region data and
+    psi_in from the library's generator (R = 2 nx ny nz)."""
+    off, ln, rg, zz = trace(geometry, T, seed, device)
+    R = geometry.regions
+    sig, src = np.empty((R, G), np.float32), np.empty((R, F, G), np.float32)
+    _check(lib().smoc_generate_regions(seed, G, R, F, sig.ravel(), src.ravel()),
+           "smoc_generate_regions")
+    psi = np.empty((T, G), np.float32)
+    _check(lib().smoc_generate_psi(seed, T, G, psi.ravel()), "smoc_generate_psi")
+    return Problem(G, R, F, sig, src, ln, rg, zz, off, psi, seed)

@@ -86,3 +86,22 @@ def test_driver_checks_against_oracle_results_file(gpu_ready, tmp_path):
     # a tolerance below the FP32 rounding of psi_out in the file fails with status 3
     r = run("--load-problem", ppath, "--check-results", rpath, "--tol", "1e-12")
     assert r.returncode == 3 and "FAILED" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_traced_stream_and_outer_iterations(gpu_ready):
+    """--trace builds the stream by ray tracing; the checksums equal the same
+    problem built through the API; --outer runs the source iteration after."""
+    geo = smoc.Geometry(12, 10, 6, 1.26, 0.54, 60.0)
+    r = run("--trace", "4000", "--geometry", "12,10,6,1.26,0.54,60", "-e", "64",
+            "--tally", "det-f64", "--seed", "9", "--outer", "2", "--scatter", "0.3")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert "Traced geometry:" in r.stdout and "OUTER ITERATION" in r.stdout
+    lines = [l for l in r.stdout.splitlines() if "checksum:" in l]
+    drv = float(lines[0].split(":")[1])  # fine_flux of the first sweep
+    prob = smoc.traced_problem(geo, 4000, G=64, F=5, seed=9)
+    with smoc.Context(64, geo.regions, 5, tally="det-f64") as ctx:
+        ctx.load(prob)
+        ctx.run()
+        f, _ = ctx.results()
+    assert drv == smoc.checksums(f, np.zeros((0, 64), np.float32))[0]

@@ -287,3 +287,55 @@ def test_update_zero_scatter_and_linearity():
     np.testing.assert_allclose(d1, 2 * d2, rtol=1e-12)
     empty = vol == 0
     assert np.all(d1[empty] == 0)
+
+
+# ---- tracer restatement ------------------------------------------------------------
+def _geometry(nx=6, ny=5, nz=4, pitch=1.26, r=0.54, height=7.0):
+    return pyoracle.Geometry(nx, ny, nz, pitch, r, height)
+
+
+def test_trace_lengths_add_up_to_the_chord():
+    g = _geometry()
+    off, ln, rg, zz = pyoracle.trace(g, 7, 300)
+    for t in range(300):
+        ray = pyoracle.trace_ray(g, 7, t)
+        # the chord ends on the box surface: an independent check of L
+        ex, ey, ez = ray.ex + ray.L * ray.dx, ray.ey + ray.L * ray.dy, ray.ez + ray.L * ray.dz
+        X, Y, Z = g.nx * g.pitch, g.ny * g.pitch, g.height
+        on_face = [abs(v) < 1e-9 or abs(v - hi) < 1e-9
+                   for v, hi in ((ex, X), (ey, Y), (ez, Z))]
+        assert any(on_face)
+        assert abs(ray.dx ** 2 + ray.dy ** 2 + ray.dz ** 2 - 1) < 1e-12
+        seg = ln[off[t]:off[t + 1]].astype(np.float64)
+        assert abs(seg.sum() - ray.L) <= 1e-6 * max(ray.L, 1e-3)
+
+
+def test_trace_regions_and_z_by_brute_force():
+    g = _geometry()
+    off, ln, rg, zz = pyoracle.trace(g, 11, 200)
+    R = 2 * g.nx * g.ny * g.nz
+    assert rg.min() >= 0 and rg.max() < R and zz.min() >= 0 and zz.max() < 1
+    dzl = g.height / g.nz
+    for t in range(0, 200, 7):
+        ray = pyoracle.trace_ray(g, 11, t)
+        s = 0.0
+        for k in range(off[t], off[t + 1]):
+            m = s + 0.5 * float(ln[k])
+            px, py, pz = ray.ex + m * ray.dx, ray.ey + m * ray.dy, ray.ez + m * ray.dz
+            ix, iy = int(px // g.pitch), int(py // g.pitch)
+            cx, cy = (ix + 0.5) * g.pitch, (iy + 0.5) * g.pitch
+            outside = 0 if (px - cx) ** 2 + (py - cy) ** 2 < g.pin_radius ** 2 else 1
+            layer = min(int(pz // dzl), g.nz - 1)
+            if float(ln[k]) > 1e-6:  # skip slivers at faces, where rounding decides
+                assert rg[k] == ((iy * g.nx + ix) * 2 + outside) * g.nz + layer, (t, k)
+                assert abs(zz[k] - (pz - layer * dzl) / dzl) < 1e-5
+            s += float(ln[k])
+
+
+def test_trace_pin_fraction_is_the_volume_fraction():
+    g = _geometry(nx=10, ny=10, nz=3, height=12.6)
+    off, ln, rg, zz = pyoracle.trace(g, 3, 3000)
+    inside = ((rg // g.nz) % 2) == 0
+    frac = ln[inside].astype(np.float64).sum() / ln.astype(np.float64).sum()
+    want = np.pi * g.pin_radius ** 2 / g.pitch ** 2
+    assert abs(frac - want) < 0.02, (frac, want)
