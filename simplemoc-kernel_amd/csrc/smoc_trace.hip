@@ -241,9 +241,9 @@ template <typename T> struct Buf {
 
 int check_args(int device, const smoc_geometry *g, int64_t T, const char *what)
 {
-    if (!geometry_ok(g) || T < 0 || T > ((int64_t)1 << 40)) {
+    if (!geometry_ok(g) || T < 0 || T > (int64_t)INT32_MAX) {
         smoc_set_error("%s: need nx, ny, nz >= 1, pitch > 0, 0 <= pin_radius < pitch / 2, "
-                       "height > 0, 2 nx ny nz < 2^29 and 0 <= T", what);
+                       "height > 0, 2 nx ny nz < 2^29 and 0 <= T < 2^31", what);
         return SMOC_ERR_ARG;
     }
     int n = 0;
