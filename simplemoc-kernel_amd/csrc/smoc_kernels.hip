@@ -450,45 +450,54 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 // attenuate_run() for a segment on which every lane of the wave has tau < 1
 // (ACCURATE exp, checked once per segment on the scalar unit): the series
 // alone, with tau^2 shared by the series and reuse, and poly / 6 = tau^4 r4.
-// HAS_C2 = false: a first or last fine interval, whose fit is linear (c2 =
-// 0, so q2 = 0 and q1 = c1): the terms in c2 are exactly zero and skipped,
-// with the same results.
-template <int MODE, bool HAS_C2>
-__device__ __forceinline__ float attenuate_series(const RunK &k, float ds, float zin, float zin2,
-                                                  float &psi, float &one_m_ev)
+// Split in two: series_a holds everything that does not depend on psi, so a
+// wave can evaluate it for two segments before the psi updates run one after
+// the other (series_b).  HAS_C2 = false: a first or last fine interval, whose
+// fit is linear (c2 = 0, so q2 = 0 and q1 = c1): the terms in c2 are exactly
+// zero and skipped, with the same results.
+struct SegA {
+    float e1, ev;   // ev / sigT, 1 - e^-tau
+    float fpart;    // the flux integral without its psi term
+    float src;      // the source part S of psi' = S + psi (1 - ev)
+};
+
+template <bool HAS_C2>
+__device__ __forceinline__ SegA series_a(const RunK &k, float ds, float zin, float zin2)
 {
     const float tau = k.sig * ds;
     const float r4 = r4_series(tau);
     const float t2 = tau * tau;
     const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
     const float tme = t2 * r;  // tau - ev
-    const float ev = tau - tme;
-    const float e1 = ev * k.i1;
+    SegA a;
+    a.ev = tau - tme;
+    a.e1 = a.ev * k.i1;
     const float m2 = tme * k.i2;
-    const float reuse = (t2 - 2.f * tau) + e1 * k.i2x2;
-    float flux_integral, src;
+    const float reuse = (t2 - 2.f * tau) + a.e1 * k.i2x2;
     if constexpr (HAS_C2) {
         const float p6 = (t2 * t2) * r4;
         const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
         const float q1m = k.a1 + k.b1 * zin;
-        flux_integral = q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * p6;
-        src = q0 * e1 + q1m * m2 + k.q2m * reuse;
+        a.fpart = q0 * m2 + q1m * reuse + k.kfi6 * p6;
+        a.src = q0 * a.e1 + q1m * m2 + k.q2m * reuse;
     } else {
         const float q0 = k.c0 + k.c1 * zin;
-        flux_integral = q0 * m2 + psi * e1 + k.a1 * reuse;
-        src = q0 * e1 + k.a1 * m2;
+        a.fpart = q0 * m2 + k.a1 * reuse;
+        a.src = q0 * a.e1 + k.a1 * m2;
     }
-    if constexpr (MODE == MODE_XFER)
-        one_m_ev = 1.f - ev;
-    // psi' = S + psi (1 - ev) with S the source terms: the small change
-    // S - psi ev is formed first (one FMA), then added to psi, so only one
-    // rounding happens at the magnitude of psi (a run near its steady state
-    // psi = S / ev would otherwise accumulate the roundings of every term)
-    psi = psi + (src - psi * ev);
-    return flux_integral;
+    return a;
 }
 
-
+// psi' = S + psi (1 - ev) as psi + (S - psi ev): the small change is formed
+// first (one FMA), then added to psi, so only one rounding happens at the
+// magnitude of psi (a run near its steady state psi = S / ev would otherwise
+// accumulate the roundings of every term).  Returns the flux integral.
+__device__ __forceinline__ float series_b(const SegA &a, float &psi)
+{
+    const float flux_integral = a.fpart + psi * a.e1;
+    psi = psi + (a.src - psi * a.ev);
+    return flux_integral;
+}
 
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
 __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
@@ -652,36 +661,64 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                 // once per run (a linear fit drops its zero c2 terms)
                 auto walk = [&](auto has_c2) {
                     constexpr bool HAS_C2 = decltype(has_c2)::value;
+                    auto series_ok = [&](const RawSeg &sg) {
+                        return EXPM == SMOC_EXP_ACCURATE && __float_as_uint(sg.len) < thr;
+                    };
+                    auto tally = [&](int j, float fi, float ev) {
+                        if constexpr (MODE == MODE_XFER)
+                            mprod[j] *= 1.f - ev;
+                        else
+                            acc[j].add(fi);
+                    };
                     for (;;) {
                         RawSeg nx = load_raw(len, code, zin, k + 1);
                         nx.code = k + 1 < ke32 ? nx.code : -1;
-                        const float zin2 = HAS_C2 ? cur.zin * cur.zin : 0.f;
-                        if (EXPM == SMOC_EXP_ACCURATE && __float_as_uint(cur.len) < thr) {
+                        if (series_ok(cur)) {
+                            // two segments of the run at once when both take
+                            // the series: their psi-free parts are independent
+                            RawSeg n2 = load_raw(len, code, zin, k + 2);
+                            n2.code = k + 2 < ke32 ? n2.code : -1;
+                            const bool pair = nx.code == ccode && series_ok(nx);
+                            const float z0 = HAS_C2 ? cur.zin * cur.zin : 0.f;
+                            SegA a0[GPL];
 #pragma unroll
-                            for (int j = 0; j < GPL; j++) {
-                                float m;
-                                const float tally = attenuate_series<MODE, HAS_C2>(
-                                    rk[j], cur.len, cur.zin, zin2, psi[j], m);
-                                if constexpr (MODE == MODE_XFER)
-                                    mprod[j] *= m;
-                                else
-                                    acc[j].add(tally);
+                            for (int j = 0; j < GPL; j++)
+                                a0[j] = series_a<HAS_C2>(rk[j], cur.len, cur.zin, z0);
+                            if (pair) {
+                                const float z1 = HAS_C2 ? nx.zin * nx.zin : 0.f;
+                                SegA a1[GPL];
+#pragma unroll
+                                for (int j = 0; j < GPL; j++)
+                                    a1[j] = series_a<HAS_C2>(rk[j], nx.len, nx.zin, z1);
+#pragma unroll
+                                for (int j = 0; j < GPL; j++) {
+                                    tally(j, series_b(a0[j], psi[j]), a0[j].ev);
+                                    tally(j, series_b(a1[j], psi[j]), a1[j].ev);
+                                }
+                                k += 2;
+                                cur = n2;
+                            } else {
+#pragma unroll
+                                for (int j = 0; j < GPL; j++)
+                                    tally(j, series_b(a0[j], psi[j]), a0[j].ev);
+                                ++k;
+                                cur = nx;
                             }
                         } else {
                             const float zin2g = cur.zin * cur.zin;
 #pragma unroll
                             for (int j = 0; j < GPL; j++) {
                                 float m;
-                                const float tally = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
-                                                                        zin2g, psi[j], m, tb);
+                                const float fi = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
+                                                                     zin2g, psi[j], m, tb);
                                 if constexpr (MODE == MODE_XFER)
                                     mprod[j] *= m;
                                 else
-                                    acc[j].add(tally);
+                                    acc[j].add(fi);
                             }
+                            ++k;
+                            cur = nx;
                         }
-                        ++k;
-                        cur = nx;
                         if (cur.code != ccode)
                             break;
                     }
