@@ -44,6 +44,9 @@ constexpr float kInv2Dz2 = 1.0f / (2.f * kDz * kDz);
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// segments of a sorted run evaluated together by the series path
+constexpr int kSeries = 8;
+
 // s_waitcnt immediate (gfx9 encoding) that waits for lgkmcnt = 0 only:
 // vmcnt 63 (bits 3:0 and 15:14), expcnt 7 (bits 6:4), lgkmcnt 0 (bits 11:8)
 constexpr int kWaitLgkm0 = 0xC07F;
@@ -671,39 +674,49 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             acc[j].add(fi);
                     };
                     for (;;) {
-                        RawSeg nx = load_raw(len, code, zin, k + 1);
-                        nx.code = k + 1 < ke32 ? nx.code : -1;
+                        // the next kSeries segments (the arrays are padded, so
+                        // the loads past the item's end are in bounds)
+                        RawSeg nx[kSeries];
+#pragma unroll
+                        for (int u = 0; u < kSeries; u++) {
+                            nx[u] = load_raw(len, code, zin, k + 1 + u);
+                            nx[u].code = k + 1 + u < ke32 ? nx[u].code : -1;
+                        }
                         if (series_ok(cur)) {
-                            // two segments of the run at once when both take
-                            // the series: their psi-free parts are independent
-                            RawSeg n2 = load_raw(len, code, zin, k + 2);
-                            n2.code = k + 2 < ke32 ? n2.code : -1;
-                            const bool pair = nx.code == ccode && series_ok(nx);
-                            const float z0 = HAS_C2 ? cur.zin * cur.zin : 0.f;
-                            SegA a0[GPL];
+                            // up to kSeries segments of the run at once when
+                            // all take the series: their psi-free parts are
+                            // independent of each other
+                            int n = 1;
+                            while (n < kSeries && nx[n - 1].code == ccode && series_ok(nx[n - 1]))
+                                ++n;
+                            auto block = [&](auto cnt) {
+                                constexpr int N = decltype(cnt)::value;
+                                SegA a[N][GPL];
 #pragma unroll
-                            for (int j = 0; j < GPL; j++)
-                                a0[j] = series_a<HAS_C2>(rk[j], cur.len, cur.zin, z0);
-                            if (pair) {
-                                const float z1 = HAS_C2 ? nx.zin * nx.zin : 0.f;
-                                SegA a1[GPL];
+                                for (int u = 0; u < N; u++) {
+                                    const RawSeg &sg = u == 0 ? cur : nx[u - 1];
+                                    const float z2 = HAS_C2 ? sg.zin * sg.zin : 0.f;
 #pragma unroll
-                                for (int j = 0; j < GPL; j++)
-                                    a1[j] = series_a<HAS_C2>(rk[j], nx.len, nx.zin, z1);
-#pragma unroll
-                                for (int j = 0; j < GPL; j++) {
-                                    tally(j, series_b(a0[j], psi[j]), a0[j].ev);
-                                    tally(j, series_b(a1[j], psi[j]), a1[j].ev);
+                                    for (int j = 0; j < GPL; j++)
+                                        a[u][j] = series_a<HAS_C2>(rk[j], sg.len, sg.zin, z2);
                                 }
-                                k += 2;
-                                cur = n2;
-                            } else {
 #pragma unroll
                                 for (int j = 0; j < GPL; j++)
-                                    tally(j, series_b(a0[j], psi[j]), a0[j].ev);
-                                ++k;
-                                cur = nx;
-                            }
+#pragma unroll
+                                    for (int u = 0; u < N; u++)
+                                        tally(j, series_b(a[u][j], psi[j]), a[u][j].ev);
+                                k += N;
+                                cur = nx[N - 1];
+                            };
+                            // the largest power of two <= n (and <= kSeries)
+                            if (kSeries >= 8 && n >= 8)
+                                block(std::integral_constant<int, (kSeries >= 8 ? 8 : 1)>{});
+                            else if (kSeries >= 4 && n >= 4)
+                                block(std::integral_constant<int, (kSeries >= 4 ? 4 : 1)>{});
+                            else if (n >= 2)
+                                block(std::integral_constant<int, 2>{});
+                            else
+                                block(std::integral_constant<int, 1>{});
                         } else {
                             const float zin2g = cur.zin * cur.zin;
 #pragma unroll
@@ -717,7 +730,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                     acc[j].add(fi);
                             }
                             ++k;
-                            cur = nx;
+                            cur = nx[0];
                         }
                         if (cur.code != ccode)
                             break;
