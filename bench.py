@@ -11,7 +11,9 @@ before timing starts.  `value` is segment-group attenuations/s of that step.
 Also measured, same shape:
   * `mixed`: the batched mixed-precision path -- the stream pre-sorted by
     (region, fine interval), BF16 storage of sigT / fine_source, FP32 math,
-    runs summed in registers and a deterministic FP64 segmented reduction;
+    runs summed in registers and a deterministic FP64 segmented reduction,
+    launched as the autotuner picks it (during the warm-up steps) and replayed
+    from a captured hipGraph;
   * `roofline`: algorithmic bytes per segment-group of the dominant kernel
     (k_sweep) over its mean duration from HIP events on the library stream;
     `traffic` comes from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -256,14 +258,16 @@ def main():
         mprob = smoc.generate(S, G=G, R=R, F=F, seed=42, track_len=L, sorted_stream=True)
         with smoc.Context(G, R, F, storage=mstore, exp=args.exp, tally=mtally) as ctx:
             ctx.load(mprob)
-            mwall, msweep, mtotal, mused = timed_steps(ctx, args.steps, args.warmup, {})
+            mwall, msweep, mtotal, mused = timed_steps(ctx, args.steps, max(args.warmup, 1),
+                                                       {"autotune": 1, "use_graph": 1})
             mf, mp = ctx.results()
         mvalue = sg / (mwall / args.steps)
         mB = alg_bytes_per_sg(mstore, mtally, G, L)
         out["mixed"] = {
             "workload": "same shape; stream pre-sorted by (region, fine interval), tracks of "
                         f"{L} consecutive sorted segments; {mstore} storage, FP32 math, runs "
-                        "summed in registers, deterministic FP64 segmented reduction",
+                        "summed in registers, deterministic FP64 segmented reduction; "
+                        "autotuned launch replayed from a hipGraph",
             "value": mvalue, "ms_per_step": mwall / args.steps * 1e3,
             "vs_plain": mvalue / value, "sweep_ms": msweep, "total_ms": mtotal,
             "hbm_gbs_alg": mB * sg / (mtotal * 1e-3) / 1e9, "alg_bytes_per_sg": mB,
