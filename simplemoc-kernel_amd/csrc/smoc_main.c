@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "smoc.h"
 
@@ -102,6 +103,13 @@ static int parse_choice(const char *v, const char *const *names, int n)
             return i;
     print_CLI_error();
     return -1;
+}
+
+static double wall_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
 /* read_CLI (src/cuda/io.cu:107-162): unknown options print usage and exit(1) */
@@ -233,15 +241,18 @@ int main(int argc, char *argv[])
 
     int64_t S = o.in.segments, T = 0;
     int64_t *track_off = NULL;
+    double trace_s = 0.0;
     if (o.trace_tracks > 0 && !o.load_problem) {
         R = 2 * o.geo.nx * o.geo.ny * o.geo.nz;
         T = o.trace_tracks;
         track_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(T + 1));
         if (!track_off)
             die("track allocation", SMOC_ERR_NOMEM);
+        const double t0 = wall_s();
         int rc = smoc_trace_count(o.device, &o.geo, o.seed, T, track_off, &S);
         if (rc)
             die("smoc_trace_count", rc);
+        trace_s = wall_s() - t0;
         o.in.segments = S;
     } else if (o.load_problem) {
         T = pinfo.T;
@@ -331,9 +342,13 @@ int main(int argc, char *argv[])
         if ((rc = smoc_generate_regions(o.seed, G, R, F, sigT, src)))
             die("smoc_generate_regions", rc);
         if (o.trace_tracks > 0) {
+            const double t0 = wall_s();
             if ((rc = smoc_trace_segments(o.device, &o.geo, o.seed, T, track_off, len, region,
                                           z)))
                 die("smoc_trace_segments", rc);
+            trace_s += wall_s() - t0;
+            printf("Ray tracing: %.3f s for %lld tracks (count and fill passes, with copies)\n",
+                   trace_s, (long long)T);
         } else if ((rc = smoc_generate_segments(o.seed, S, R, len, region, z))) {
             die("smoc_generate_segments", rc);
         }
