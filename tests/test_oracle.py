@@ -339,3 +339,18 @@ def test_trace_pin_fraction_is_the_volume_fraction():
     frac = ln[inside].astype(np.float64).sum() / ln.astype(np.float64).sum()
     want = np.pi * g.pin_radius ** 2 / g.pitch ** 2
     assert abs(frac - want) < 0.02, (frac, want)
+
+
+def test_glibc_expf_within_one_ulp():
+    """The oracle's native exponential is glibc's expf (src/cpu/kernel.c:221
+    calls expf).  Pinned against FP64 exp: at most 1 ulp over tau in [0, 20]."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.expf.argtypes = [ctypes.c_float]
+    libm.expf.restype = ctypes.c_float
+    xs = np.linspace(-20.0, 0.0, 4001).astype(np.float32)
+    got = np.array([libm.expf(float(x)) for x in xs], np.float32)
+    want = np.exp(xs.astype(np.float64))
+    ulp = np.spacing(want.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got.astype(np.float64) - want) / ulp) <= 1.0
