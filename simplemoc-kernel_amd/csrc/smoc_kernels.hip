@@ -628,11 +628,12 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         // Region-sorted stream: a track stays in one (region, interval) for
         // many segments.  The outer loop starts a run: it gathers the tuple,
         // derives the run constants and, at the end, flushes the run's tally
-        // summed in registers.  The inner loop walks the run with as little
-        // scalar work per segment as possible: one scalar load per stream
-        // array, issued a segment ahead (the device arrays are padded, so the
-        // load at the item's end is in bounds and only marked past-the-end),
-        // one compare for the exp branch and one for the run's end.
+        // summed in registers.  The inner loop walks the run in blocks: the
+        // stream entries of the next kSeries segments are loaded ahead (the
+        // device arrays are padded, so loads past the item's end are in bounds
+        // and only marked past-the-end), and up to kSeries consecutive
+        // segments that all take the series branch have their psi-free parts
+        // evaluated together before their psi updates run in order.
         if (kb32 < ke32) {
             RunK rk[GPL];
             uint32_t k = kb32;
@@ -922,7 +923,7 @@ __global__ void __launch_bounds__(256) k_pack(smoc_pack_args a)
     }
 }
 
-// Series threshold of each row (region, interval), for attenuate_series: with
+// Series threshold of each row (region, interval), for the series branch: with
 // ds below it, sigT ds < 1 for every group of the row.  thr = (1 - 2^-20) /
 // max_g sigT (IEEE division, sigT as the sweep unpacks it) gives sigT ds <
 // (1 - 2^-20)(1 + 2^-24) < 1 - 2^-21 before rounding, so the FP32 product
