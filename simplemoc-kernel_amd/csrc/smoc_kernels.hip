@@ -552,7 +552,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 
     const raw_t *__restrict__ tup = static_cast<const raw_t *>(a.tuples);
     // segment indices fit 32 bits (S <= 2^31 - 1 is checked on upload), so
-    // the loop bounds are 32-bit scalar compares; k + 2 cannot wrap unsigned
+    // the loop bounds are 32-bit scalar compares; k + kSeries cannot wrap
+    // unsigned
     const uint32_t kb32 = (uint32_t)kb, ke32 = (uint32_t)ke;
     auto raw_at = [&](uint32_t k) {
         return k < ke32 ? load_raw(len, code, zin, k) : RawSeg{0.f, 0.f, -1};
