@@ -117,3 +117,21 @@ def test_launch_shapes_vs_oracle(gpu_ready, ragged, ragged_oracle, gpl, wpb, chu
     print(f"  gpl {gpl} wpb {wpb} chunk {chunk} {tally}: normwise {ef:.2e} / {ep:.2e}; "
           f"worst flux entry {bad}", flush=True)
     assert ef < 1e-5 and ep < 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_fixed64_chunked_graph(gpu_ready, ragged, ragged_oracle, storage):
+    """FIXED64 with chunked tracks, the persistent kernel and graph replays:
+    bit-exact run to run (integer atomics), at the oracle bar."""
+    print(f"case fixed64 chunked {storage}", flush=True)
+    with smoc.Context(ragged.G, ragged.R, ragged.F, storage=storage, tally="fixed64") as ctx:
+        ctx.load(ragged)
+        ctx.run(chunk=512, persistent=1, use_graph=1)
+        f1, p1 = ctx.results()
+        ctx.run(chunk=512, persistent=1, use_graph=1)
+        f2, p2 = ctx.results()
+    assert np.array_equal(f1, f2) and np.array_equal(p1, p2)
+    from conftest import quantized
+    of, op = (ragged_oracle if storage == "f32" else
+              pyoracle.sweep_f64(quantized(ragged, storage), pyoracle.EXP_NATIVE, nthreads=8))
+    assert normwise(f1, of) < 1e-5 and normwise(p1, op) < 1e-5
