@@ -169,12 +169,18 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
             "fine_flux_checksum_rel": abs(f.sum() - of.sum()) / abs(of.sum()),
             "state_flux_checksum_rel": abs(p.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
         if mixed is not None:
-            mprob, mf, mp, storage = mixed
+            mprob, mf, mp, storage, f32_run = mixed
             of, op = pyoracle.sweep_f64(mprob, pyoracle.EXP_NATIVE, nthreads=threads)
             acc["mixed_vs_fp64_on_fp32_masters"] = {
                 "fine_flux_checksum_rel": abs(mf.sum() - of.sum()) / abs(of.sum()),
                 "state_flux_checksum_rel":
                     abs(mp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+            if f32_run is not None:
+                ff, fp = f32_run
+                acc["mixed_f32_storage_vs_fp64"] = {
+                    "fine_flux_checksum_rel": abs(ff.sum() - of.sum()) / abs(of.sum()),
+                    "state_flux_checksum_rel":
+                        abs(fp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
             sys.path.insert(0, os.path.join(ROOT, "tests"))
             from conftest import quantized
             of, op = pyoracle.sweep_f64(quantized(mprob, storage), pyoracle.EXP_NATIVE,
@@ -273,7 +279,20 @@ def main():
             "hbm_gbs_alg": mB * sg / (mtotal * 1e-3) / 1e9, "alg_bytes_per_sg": mB,
             "launch": mused,
             "checksums": dict(zip(("fine_flux", "state_flux"), smoc.checksums(mf, mp)))}
-        mixed = (mprob, mf, mp, mstore)
+        # the same sorted path with FP32 storage: no quantisation, so its error
+        # against FP64 on the FP32 masters is the arithmetic's alone
+        with smoc.Context(G, R, F, storage="f32", exp=args.exp, tally=mtally) as ctx:
+            ctx.load(mprob)
+            fwall, fsweep, ftotal, fused = timed_steps(ctx, args.steps, max(args.warmup, 1),
+                                                       {"autotune": 1, "use_graph": 1})
+            ff, fp = ctx.results()
+        fvalue = sg / (fwall / args.steps)
+        out["mixed_f32_storage"] = {
+            "workload": "as mixed, with FP32 storage of sigT / fine_source",
+            "value": fvalue, "ms_per_step": fwall / args.steps * 1e3,
+            "vs_plain": fvalue / value, "sweep_ms": fsweep, "total_ms": ftotal,
+            "launch": fused}
+        mixed = (mprob, mf, mp, mstore, (ff, fp))
 
     if not args.no_cpu_baseline:
         base, acc = cpu_leg(prob, args.cpu_sample_segments, args.cpu_single_segments,
