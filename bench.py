@@ -13,7 +13,9 @@ Also measured, same shape:
     (region, fine interval), BF16 storage of sigT / fine_source, FP32 math,
     runs summed in registers and a deterministic FP64 segmented reduction,
     launched as the autotuner picks it (during the warm-up steps) and replayed
-    from a captured hipGraph;
+    from a captured hipGraph; `mixed_f32_storage` is the same path with FP32
+    storage, whose error against FP64 on the FP32 masters is the arithmetic's
+    alone;
   * `roofline`: algorithmic bytes per segment-group of the dominant kernel
     (k_sweep) over its mean duration from HIP events on the library stream;
     `traffic` comes from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE
