@@ -20,6 +20,9 @@ Also measured, same shape:
     (k_sweep) over its mean duration from HIP events on the library stream;
     `traffic` comes from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary committed under profiles/ for this workload;
+    `atomic_tally_floor` times the same stream's per-segment FP32 atomic
+    flushes alone (k_tally_probe: no gathers, no arithmetic), the part of
+    the sweep that no data layout removes;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
     on this host's cores on a bounded sample of the same stream, plus the
     accuracy of both GPU paths against the FP64 oracle (the checker).
@@ -232,6 +235,16 @@ def main():
     traffic = measured_traffic() if (args.storage, args.tally, S, G) == \
         ("f32", "atomic-f32", 50_000_000, 100) else None
 
+    floor = None
+    if args.tally == "atomic-f32":
+        # the same stream's per-segment FP32 atomic flushes alone: the floor
+        # that the tally sets for k_sweep on a stream in generation order
+        fms = smoc.probe_tally(S, R * F, G, reps=5)
+        floor = {"kernel": "k_tally_probe", "ms": fms,
+                 "atomic_requests_64B": S * -(-G // 16),
+                 "requests_per_s": S * -(-G // 16) / (fms * 1e-3),
+                 "sweep_over_floor": sweep_ms / fms}
+
     out = {
         "metric": "segment-group attenuations/s",
         "value": value,
@@ -253,7 +266,8 @@ def main():
                    "tally": args.tally, "launch": used},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep", "kernel_ms": sweep_ms, "alg_bytes_per_sg": B},
+                     "kernel": "k_sweep", "kernel_ms": sweep_ms, "alg_bytes_per_sg": B,
+                     "atomic_tally_floor": floor},
         "hbm_gbs": achieved,
         "checksums": {"fine_flux": ck_f, "state_flux": ck_p},
         "device": f"{dev_name}, {cus} CUs",

@@ -247,6 +247,15 @@ int smoc_device_info(int device, char *name, int name_cap, int *compute_units);
  * counters and to state the achievable copy bandwidth next to the roofline. */
 int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms);
 
+/* Measurement utility (no reference counterpart): the unsorted sweep's
+ * per-segment FP32 atomic tally alone -- S segments of 100 per wavefront, each
+ * adding 1.0 to the G groups of a pseudo-random row of a [rows][G padded to
+ * 128] array, with no gathers and no arithmetic.  Returns the mean kernel time
+ * of `reps` launches (after one warm-up): the floor that the atomic tally sets
+ * for k_sweep on a stream in generation order.  Needs 1 <= G <= 65536,
+ * 1 <= rows, 0 <= S, reps >= 1. */
+int smoc_probe_tally(int device, int64_t S, int rows, int G, int reps, double *mean_ms);
+
 int smoc_create(smoc_ctx **ctx, const smoc_params *p);
 int smoc_destroy(smoc_ctx *ctx);
 

@@ -245,6 +245,53 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
     return SMOC_OK;
 }
 
+int smoc_probe_tally(int device, int64_t S, int rows, int G, int reps, double *mean_ms)
+{
+    if (S < 0 || rows < 1 || G < 1 || G > 65536 || reps < 1 || !mean_ms) {
+        smoc_set_error("smoc_probe_tally: need S >= 0, rows >= 1, 1 <= G <= 65536, reps >= 1");
+        return SMOC_ERR_ARG;
+    }
+    const int64_t nblk = ((S + 99) / 100 + 3) / 4;
+    if (nblk > INT32_MAX) {
+        smoc_set_error("smoc_probe_tally: %lld segments is too many", (long long)S);
+        return SMOC_ERR_ARG;
+    }
+    int ndev = 0;
+    int rc = smoc_device_count(&ndev);
+    if (rc != SMOC_OK)
+        return rc;
+    if (device < 0 || device >= ndev) {
+        smoc_set_error("smoc_probe_tally: device %d not available", device);
+        return SMOC_ERR_NODEV;
+    }
+    HIPCHK(hipSetDevice(device));
+    const int Gp = round_up(G, 128);
+    DevBuf<float> flux;
+    rc = flux.alloc((size_t)rows * Gp);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemset(flux.p, 0, sizeof(float) * (size_t)rows * Gp));
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int grid = (int)(nblk > 0 ? nblk : 1);
+    LAUNCHCHK(smoc_launch_tally_probe(flux.p, S, rows, G, Gp, grid, s));  // warm-up
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++)
+        LAUNCHCHK(smoc_launch_tally_probe(flux.p, S, rows, G, Gp, grid, s));
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *mean_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return SMOC_OK;
+}
+
 int smoc_create(smoc_ctx **out, const smoc_params *p)
 {
     if (!out || !p) {

@@ -1067,6 +1067,29 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
             dst[base + u * 256] = v[u];
 }
 
+// Tally-only probe: the per-segment FP32 atomic flush of the sweep on an
+// unsorted stream, without the gathers and the arithmetic.  Wave w flushes
+// segments [w*100, w*100 + 100) of S, segment s into row mix(s) % rows of a
+// [rows][Gp] array, one atomic per group with the lanes past G masked, as the
+// sweep does; what it measures is the memory-side atomic rate for that
+// pattern, the floor of the unsorted sweep with an atomic tally.
+__global__ void __launch_bounds__(256) k_tally_probe(float *flux, int64_t S, int rows, int G,
+                                                      int Gp)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t s0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 100;
+    const int64_t s1 = s0 + 100 < S ? s0 + 100 : S;
+    for (int64_t s = s0; s < s1; s++) {
+        uint32_t h = (uint32_t)s * 0x9E3779B1u;  // Fibonacci hashing, then a xorshift-multiply
+        h ^= h >> 15;
+        h *= 0x2C1B3C6Du;
+        h ^= h >> 12;
+        float *row = flux + (int64_t)(h % (uint32_t)rows) * Gp;
+        for (int g = lane; g < G; g += 64)
+            unsafeAtomicAdd(row + g, 1.0f);
+    }
+}
+
 // FIXED64 result, in place: int64 fixed point -> init + value as FP64.
 __global__ void __launch_bounds__(256) k_fixed_to_f64(int64_t *flux, const float *init, int64_t n)
 {
@@ -1284,6 +1307,16 @@ extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int gri
         return 0;
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_tally_probe(void *flux, int64_t S, int rows, int G, int Gp, int grid,
+                                       void *stream)
+{
+    if (S == 0)
+        return 0;
+    hipLaunchKernelGGL(k_tally_probe, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<float *>(flux), S, rows, G, Gp);
     return (int)hipGetLastError();
 }
 

@@ -132,6 +132,7 @@ SIGNATURES = {
     "smoc_device_count": (_i, [ctypes.POINTER(_i)]),
     "smoc_device_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i)]),
     "smoc_probe_copy": (_i, [_i, _i64, _i, ctypes.POINTER(_d)]),
+    "smoc_probe_tally": (_i, [_i, _i64, _i, _i, _i, ctypes.POINTER(_d)]),
     "smoc_create": (_i, [ctypes.POINTER(_P), ctypes.POINTER(Params)]),
     "smoc_destroy": (_i, [_P]),
     "smoc_set_regions": (_i, [_P, _F32, _F32, _P]),
@@ -211,6 +212,14 @@ def probe_copy(nbytes: int, reps: int = 5, device: int = 0) -> float:
     """Mean time (ms) of the plain copy kernel over nbytes (read + write)."""
     ms = _d()
     _check(lib().smoc_probe_copy(device, nbytes, reps, ctypes.byref(ms)), "smoc_probe_copy")
+    return ms.value
+
+
+def probe_tally(S: int, rows: int, G: int, reps: int = 5, device: int = 0) -> float:
+    """Mean time (ms) of the tally-only probe: S segments' FP32 atomic flushes
+    into pseudo-random rows of [rows][G], without gathers or arithmetic."""
+    ms = _d()
+    _check(lib().smoc_probe_tally(device, S, rows, G, reps, ctypes.byref(ms)), "smoc_probe_tally")
     return ms.value
 
 

@@ -67,3 +67,23 @@ def test_null_context_calls():
     assert L.smoc_destroy(None) == smoc.OK
     assert L.smoc_run(None, None, None) == smoc.ERR_ARG
     assert L.smoc_get_results(None, None, None) == smoc.ERR_ARG
+
+
+def test_probes_reject_bad_arguments_before_touching_a_device():
+    L = smoc.lib()
+    ms = ctypes.c_double()
+    for args in ((0, -1, 10, 100, 1), (0, 100, 0, 100, 1), (0, 100, 10, 0, 1),
+                 (0, 100, 10, 65537, 1), (0, 100, 10, 100, 0),
+                 (0, 400 * (1 << 31) * 100, 10, 100, 1)):  # grid beyond 2^31 - 1 blocks
+        assert L.smoc_probe_tally(*args, ctypes.byref(ms)) == smoc.ERR_ARG, args
+    assert L.smoc_probe_tally(0, 100, 10, 100, 1, None) == smoc.ERR_ARG
+    for args in ((0, 0, 1), (0, 24, 1), (0, 1 << 20, 0)):
+        assert L.smoc_probe_copy(*args, ctypes.byref(ms)) == smoc.ERR_ARG, args
+
+
+def test_probe_without_device_fails_loudly():
+    if smoc.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(smoc.SmocError) as e:
+        smoc.probe_tally(1000, 10, 100)
+    assert e.value.code == smoc.ERR_NODEV
