@@ -23,6 +23,8 @@ Also measured, same shape:
     `atomic_tally_floor` times the same stream's per-segment FP32 atomic
     flushes alone (k_tally_probe: no gathers, no arithmetic), the part of
     the sweep that no data layout removes;
+  * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
+    the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
     on this host's cores on a bounded sample of the same stream, plus the
     accuracy of both GPU paths against the FP64 oracle (the checker).
@@ -224,7 +226,9 @@ def main():
         launch["waves_per_block"] = args.wpb
 
     with smoc.Context(G, R, F, storage=args.storage, exp=args.exp, tally=args.tally) as ctx:
-        ctx.load(prob)
+        t0 = time.perf_counter()
+        ctx.load(prob)  # host buffers -> HBM, stream decode; returns synchronised
+        load_s = time.perf_counter() - t0
         wall, sweep_ms, total_ms, used = timed_steps(ctx, args.steps, args.warmup, launch)
         flux, psi = ctx.results()
     ck_f, ck_p = smoc.checksums(flux, psi)
@@ -269,6 +273,14 @@ def main():
                      "kernel": "k_sweep", "kernel_ms": sweep_ms, "alg_bytes_per_sg": B,
                      "atomic_tally_floor": floor},
         "hbm_gbs": achieved,
+        "handover": {
+            "load_ms": load_s * 1e3,
+            "host_bytes": int(sum(a.nbytes for a in (prob.sigT, prob.fine_source, prob.len,
+                                                     prob.region, prob.z, prob.track_off,
+                                                     prob.psi_in))),
+            "value_incl_load": sg / (load_s + wall / args.steps),
+            "note": "one upload of pageable host buffers (PCIe), the stream decode and the "
+                    "host-side track volumes, then one step; not the metric"},
         "checksums": {"fine_flux": ck_f, "state_flux": ck_p},
         "device": f"{dev_name}, {cus} CUs",
         "input_gen_s": gen_s,

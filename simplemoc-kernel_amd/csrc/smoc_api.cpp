@@ -406,16 +406,67 @@ int smoc_destroy(smoc_ctx *c)
 }
 
 // Copy a [rows][G] host array into a [rows][Gp] device array (padding = 0).
+// Rows of `cols` elements between contiguous host memory and device rows
+// `pitch` elements apart.  A 2D copy straight to or from pageable host memory
+// moves one row at a time (seconds for 5e5 rows), so the host side is one
+// contiguous copy through a device staging buffer and the strided part is a
+// device-to-device 2D copy.  Both return with the copies complete.
+extern "C++" {
+template <typename T>
+static int rows_to_device(T *dst, int64_t pitch, const T *src, int64_t rows, int cols,
+                          hipStream_t s)
+{
+    if (rows == 0)
+        return SMOC_OK;
+    const size_t n = (size_t)rows * cols;
+    if (pitch == cols) {
+        HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return SMOC_OK;
+    }
+    DevBuf<T> stage;
+    int rc = stage.alloc(n);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpyAsync(stage.p, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(T) * pitch, stage.p, sizeof(T) * cols, sizeof(T) * cols,
+                            (size_t)rows, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SMOC_OK;
+}
+
+template <typename T>
+static int rows_to_host(T *dst, const T *src, int64_t pitch, int64_t rows, int cols,
+                        hipStream_t s)
+{
+    if (rows == 0)
+        return SMOC_OK;
+    const size_t n = (size_t)rows * cols;
+    if (pitch == cols) {
+        HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return SMOC_OK;
+    }
+    DevBuf<T> stage;
+    int rc = stage.alloc(n);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipMemcpy2DAsync(stage.p, sizeof(T) * cols, src, sizeof(T) * pitch, sizeof(T) * cols,
+                            (size_t)rows, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(dst, stage.p, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return SMOC_OK;
+}
+}  // extern "C++"
+
+// host [rows][G] -> device [rows][Gp], the padding zeroed
 static int upload_padded(float *dst, const float *src, int64_t rows, int G, int Gp,
                          hipStream_t s)
 {
     if (rows == 0)
         return SMOC_OK;
     HIPCHK(hipMemsetAsync(dst, 0, sizeof(float) * (size_t)rows * Gp, s));
-    HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * Gp, src, sizeof(float) * G, sizeof(float) * G,
-                            (size_t)rows, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return SMOC_OK;
+    return rows_to_device(dst, Gp, src, rows, G, s);
 }
 
 int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
@@ -1117,22 +1168,22 @@ int smoc_get_results(smoc_ctx *c, double *fine_flux, float *psi_out)
     HIPCHK(hipSetDevice(c->p.device));
     const int G = c->p.egroups, Gp = c->Gp;
     const int64_t nrow = c->n_fsr();
+    int rc = SMOC_OK;
     if (fine_flux) {
         if (c->flux_f64()) {
-            HIPCHK(hipMemcpy2D(fine_flux, sizeof(double) * G, c->flux.p + 256, sizeof(double) * Gp,
-                               sizeof(double) * G, (size_t)nrow, hipMemcpyDeviceToHost));
+            rc = rows_to_host(fine_flux, reinterpret_cast<const double *>(c->flux.p + 256), Gp,
+                              nrow, G, c->stream);
         } else {
             std::vector<float> tmp((size_t)nrow * G);
-            HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * G, c->flux.p + 256, sizeof(float) * Gp,
-                               sizeof(float) * G, (size_t)nrow, hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < tmp.size(); i++)
+            rc = rows_to_host(tmp.data(), reinterpret_cast<const float *>(c->flux.p + 256), Gp,
+                              nrow, G, c->stream);
+            for (size_t i = 0; rc == SMOC_OK && i < tmp.size(); i++)
                 fine_flux[i] = tmp[i];
         }
     }
-    if (psi_out && c->T > 0)
-        HIPCHK(hipMemcpy2D(psi_out, sizeof(float) * G, c->psi_out.p, sizeof(float) * Gp,
-                           sizeof(float) * G, (size_t)c->T, hipMemcpyDeviceToHost));
-    return SMOC_OK;
+    if (rc == SMOC_OK && psi_out && c->T > 0)
+        rc = rows_to_host(psi_out, c->psi_out.p, Gp, c->T, G, c->stream);
+    return rc;
 }
 
 int smoc_set_scattering(smoc_ctx *c, const float *scatter, const float *q_ext)

@@ -187,20 +187,21 @@ struct EvTerms {
     float ev, one_m_ev, tme, poly;
 };
 
-// Taylor series for 0 <= tau < 1: r4 = sum_{k>=0} (-tau)^k / (k+4)!,
+// Series form for 0 <= tau < 1: r4 = sum_{k>=0} (-tau)^k / (k+4)!,
 // r = 1/2 - tau/6 + tau^2 r4; then ev = tau - tau^2 r, tme = tau^2 r and
-// poly = 6 tau^4 r4 without subtracting nearly equal numbers.  Degree 6 in
-// r4 truncates at tau^7/11! * 4! < 7e-7 relative to r4 for tau < 1, which
-// moves ev by under 3e-8 absolute.
+// poly = 6 tau^4 r4 without subtracting nearly equal numbers.  r4 is the
+// degree-4 polynomial closest to it in relative error on [0, 1] (Lawson's
+// reweighted least squares, FP32 coefficients; csrc/fit_r4.py recomputes
+// them): 2.1e-7 relative evaluated in FP32, and ev within 8e-8 absolute of
+// 1 - e^-tau over [0, 1) (the degree-6 Taylor polynomial: 1.0e-7), with two
+// fused multiply-adds fewer.
 __device__ __forceinline__ float r4_series(float tau)
 {
-    float r4 = 1.f / 3628800.f;
-    r4 = r4 * tau - 1.f / 362880.f;
-    r4 = r4 * tau + 1.f / 40320.f;
-    r4 = r4 * tau - 1.f / 5040.f;
-    r4 = r4 * tau + 1.f / 720.f;
-    r4 = r4 * tau - 1.f / 120.f;
-    return r4 * tau + 1.f / 24.f;
+    float r4 = 0x1.3cd4c4p-16f;
+    r4 = r4 * tau - 0x1.95bd10p-13f;
+    r4 = r4 * tau + 0x1.6ba232p-10f;
+    r4 = r4 * tau - 0x1.110f3ep-7f;
+    return r4 * tau + 0x1.555552p-5f;
 }
 
 __device__ __forceinline__ EvTerms ev_series(float tau)
