@@ -160,6 +160,44 @@ struct smoc_ctx {
     bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
 };
 
+// Stream and events of a measurement probe, released on every return path.
+struct ProbeTimer {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProbeTimer() = default;
+    ProbeTimer(const ProbeTimer &) = delete;
+    ProbeTimer &operator=(const ProbeTimer &) = delete;
+    ~ProbeTimer()
+    {
+        if (e0)
+            (void)hipEventDestroy(e0);
+        if (e1)
+            (void)hipEventDestroy(e1);
+        if (s)
+            (void)hipStreamDestroy(s);
+    }
+};
+
+// One warm-up launch, then `reps` launches between two events on a stream of
+// their own: the mean time per launch.
+template <typename Launch> static int time_probe(Launch launch, int reps, double *mean_ms)
+{
+    ProbeTimer t;
+    HIPCHK(hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&t.e0));
+    HIPCHK(hipEventCreate(&t.e1));
+    LAUNCHCHK(launch(t.s));
+    HIPCHK(hipEventRecord(t.e0, t.s));
+    for (int r = 0; r < reps; r++)
+        LAUNCHCHK(launch(t.s));
+    HIPCHK(hipEventRecord(t.e1, t.s));
+    HIPCHK(hipEventSynchronize(t.e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.e0, t.e1));
+    *mean_ms = ms / reps;
+    return SMOC_OK;
+}
+
 extern "C" {
 
 int smoc_device_count(int *n)
@@ -201,6 +239,11 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
         smoc_set_error("smoc_probe_copy: bytes must be a positive multiple of 16, reps >= 1");
         return SMOC_ERR_ARG;
     }
+    const int64_t nblk = (bytes / 16 + 1023) / 1024;
+    if (nblk > INT32_MAX) {
+        smoc_set_error("smoc_probe_copy: %lld bytes is too large", (long long)bytes);
+        return SMOC_ERR_ARG;
+    }
     int ndev = 0;
     int rc = smoc_device_count(&ndev);
     if (rc != SMOC_OK)
@@ -210,8 +253,6 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
         return SMOC_ERR_NODEV;
     }
     HIPCHK(hipSetDevice(device));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, device));
     DevBuf<unsigned char> a, b;
     rc = a.alloc((size_t)bytes);
     if (rc == SMOC_OK)
@@ -219,30 +260,11 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
     if (rc != SMOC_OK)
         return rc;
     HIPCHK(hipMemset(a.p, 1, (size_t)bytes));
-    hipStream_t s;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    const int64_t nblk = (bytes / 16 + 1023) / 1024;
-    if (nblk > INT32_MAX) {
-        smoc_set_error("smoc_probe_copy: %lld bytes is too large", (long long)bytes);
-        return SMOC_ERR_ARG;
-    }
+    HIPCHK(hipDeviceSynchronize());
     const int grid = (int)nblk;
-    LAUNCHCHK(smoc_launch_copy(a.p, b.p, bytes / 16, grid, s));  // warm-up
-    HIPCHK(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; r++)
-        LAUNCHCHK(smoc_launch_copy(a.p, b.p, bytes / 16, grid, s));
-    HIPCHK(hipEventRecord(e1, s));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    *mean_ms = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-    return SMOC_OK;
+    return time_probe(
+        [&](hipStream_t s) { return smoc_launch_copy(a.p, b.p, bytes / 16, grid, s); }, reps,
+        mean_ms);
 }
 
 int smoc_probe_tally(int device, int64_t S, int rows, int G, int reps, double *mean_ms)
@@ -271,25 +293,13 @@ int smoc_probe_tally(int device, int64_t S, int rows, int G, int reps, double *m
     if (rc != SMOC_OK)
         return rc;
     HIPCHK(hipMemset(flux.p, 0, sizeof(float) * (size_t)rows * Gp));
-    hipStream_t s;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipDeviceSynchronize());
     const int grid = (int)(nblk > 0 ? nblk : 1);
-    LAUNCHCHK(smoc_launch_tally_probe(flux.p, S, rows, G, Gp, grid, s));  // warm-up
-    HIPCHK(hipEventRecord(e0, s));
-    for (int r = 0; r < reps; r++)
-        LAUNCHCHK(smoc_launch_tally_probe(flux.p, S, rows, G, Gp, grid, s));
-    HIPCHK(hipEventRecord(e1, s));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    *mean_ms = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-    return SMOC_OK;
+    return time_probe(
+        [&](hipStream_t s) {
+            return smoc_launch_tally_probe(flux.p, S, rows, G, Gp, grid, s);
+        },
+        reps, mean_ms);
 }
 
 int smoc_create(smoc_ctx **out, const smoc_params *p)
