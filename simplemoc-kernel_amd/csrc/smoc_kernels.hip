@@ -677,21 +677,42 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             acc[j].add(fi);
                     };
                     for (;;) {
-                        // the next kSeries segments (the arrays are padded, so
-                        // the loads past the item's end are in bounds)
+                        // the next kSeries segments, from one 64-bit base per
+                        // array so that the scalar loads take constant offsets
+                        // (the arrays are padded, so the loads past the
+                        // item's end are in bounds; such entries are never
+                        // used: blocks are capped by the segments left)
+                        const float *__restrict__ lp = len + (size_t)k + 1;
+                        const int32_t *__restrict__ cp = code + (size_t)k + 1;
+                        const float *__restrict__ zp = zin + (size_t)k + 1;
                         RawSeg nx[kSeries];
 #pragma unroll
-                        for (int u = 0; u < kSeries; u++) {
-                            nx[u] = load_raw(len, code, zin, k + 1 + u);
-                            nx[u].code = k + 1 + u < ke32 ? nx[u].code : -1;
-                        }
+                        for (int u = 0; u < kSeries; u++)
+                            nx[u] = RawSeg{lp[u], zp[u], cp[u]};
                         if (series_ok(cur)) {
                             // up to kSeries segments of the run at once when
                             // all take the series: their psi-free parts are
-                            // independent of each other
-                            int n = 1;
-                            while (n < kSeries && nx[n - 1].code == ccode && series_ok(nx[n - 1]))
-                                ++n;
+                            // independent of each other.  Common case first,
+                            // in a few scalar operations: every look-ahead
+                            // entry continues the run below the threshold.
+                            // len bits < thr <=> thr - 1 - bits has its sign
+                            // bit clear: both are below 2^31 (len >= 0 is
+                            // checked on upload, thr > 0), and -0.0, whose
+                            // bits are 2^31, sets it as series_ok() rejects it
+                            uint32_t diff = 0, below = 0;
+#pragma unroll
+                            for (int u = 0; u < kSeries - 1; u++) {
+                                diff |= (uint32_t)(nx[u].code ^ ccode);
+                                below |= thr - 1u - __float_as_uint(nx[u].len);
+                            }
+                            uint32_t n = kSeries;
+                            if ((diff | (below & 0x80000000u)) != 0) {
+                                n = 1;
+                                while (n < kSeries && nx[n - 1].code == ccode &&
+                                       series_ok(nx[n - 1]))
+                                    ++n;
+                            }
+                            n = min(n, ke32 - k);  // segments k .. k + n - 1 are the item's
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
@@ -735,7 +756,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             ++k;
                             cur = nx[0];
                         }
-                        if (cur.code != ccode)
+                        if (k >= ke32 || cur.code != ccode)
                             break;
                     }
                 };
