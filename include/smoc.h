@@ -106,8 +106,7 @@ typedef struct {
     int seg_per_block;   /* -p: segments per work item; used as chunk when
                             chunk is 0 (0 = whole tracks)                     */
     int groups_per_lane; /* 1 or 2 energy groups per lane                     */
-    int waves_per_block; /* wavefronts per workgroup (1..16; default 4, or 8
-                            on a region-sorted stream)                        */
+    int waves_per_block; /* wavefronts per workgroup (1..16; default 4)     */
     int chunk;           /* split tracks longer than this many segments into
                             chunks joined by an affine psi scan (0 = never)    */
     int persistent;      /* 1: persistent workgroups pulling work from a queue */

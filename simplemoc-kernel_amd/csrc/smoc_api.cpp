@@ -742,10 +742,10 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
         d = *in;
     if (d.groups_per_lane != 1 && d.groups_per_lane != 2)
         d.groups_per_lane = c->p.egroups > 64 ? 2 : 1;
-    // default shape: 4 waves per workgroup; 8 on a region-sorted stream, whose
-    // run loop (blocks of up to 8 segments, 4 waves per SIMD) measured best there
+    // default shape: 4 waves per workgroup (measured best, or within 2 % of
+    // it, on unsorted and region-sorted streams alike)
     if (d.waves_per_block < 1 || d.waves_per_block > 16)
-        d.waves_per_block = c->runs ? 8 : 4;
+        d.waves_per_block = 4;
     if (d.chunk < 0)
         d.chunk = 0;
     if (d.chunk == 0 && d.seg_per_block > 0)
