@@ -411,8 +411,8 @@ struct RunK {
     float sig, i1, i2, i2x2;  // sigT, 1/sigT, 1/sigT^2, 2/sigT^2
     float c0, c1, c2;         // source fit: q0 = c0 + c1 zin + c2 zin^2
     float a1, b1;             // mu q1 = a1 + b1 zin
-    float q2m, kfi6;          // mu2 q2 and 2 mu2 q2 / sigT^4 (the integral's q2 term
-                              // is kfi6 * poly / 6)
+    float q2m, q2m2;          // mu2 q2 and 2 mu2 q2: the integral's q2 term is
+                              // (2 mu2 q2 / sigT^4) poly / 6 = 2 mu2 q2 ds^4 r4
 };
 
 template <int STORE>
@@ -427,7 +427,7 @@ __device__ __forceinline__ RunK run_consts(const float4 t, int kind)
     k.a1 = SMOC_MU * k.c1;
     k.b1 = (2.f * SMOC_MU) * k.c2;
     k.q2m = SMOC_MU2 * k.c2;
-    k.kfi6 = k.q2m * (k.i2 * k.i2 * 2.f);
+    k.q2m2 = 2.f * k.q2m;
     return k;
 }
 
@@ -444,8 +444,9 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
     const float reuse = tau * (tau - 2.f) + e1 * k.i2x2;
     const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
     const float q1m = k.a1 + k.b1 * zin;
+    const float kfi6 = k.q2m2 * (k.i2 * k.i2);  // 2 mu2 q2 / sigT^4
     const float flux_integral =
-        q0 * m2 + psi * e1 + q1m * reuse + k.kfi6 * (x.poly * (1.f / 6.f));
+        q0 * m2 + psi * e1 + q1m * reuse + kfi6 * (x.poly * (1.f / 6.f));
     psi = q0 * e1 + q1m * m2 + k.q2m * reuse + psi * x.one_m_ev;
     one_m_ev = x.one_m_ev;
     return flux_integral;
@@ -465,24 +466,26 @@ struct SegA {
     float src;      // the source part S of psi' = S + psi (1 - ev)
 };
 
+// ds2 = ds^2 and ds4 = ds^4 are the segment's (shared by every group):
+// tau - ev = tau^2 r, so (tau - ev) / sigT^2 = ds^2 r, and the q2 term of
+// the integral, (2 mu2 q2 / sigT^4) tau^4 r4, is 2 mu2 q2 ds^4 r4.
 template <bool HAS_C2>
-__device__ __forceinline__ SegA series_a(const RunK &k, float ds, float zin, float zin2)
+__device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, float ds4, float zin,
+                                         float zin2)
 {
     const float tau = k.sig * ds;
     const float r4 = r4_series(tau);
     const float t2 = tau * tau;
     const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
-    const float tme = t2 * r;  // tau - ev
     SegA a;
-    a.ev = tau - tme;
+    a.ev = tau - t2 * r;  // one fused rounding
     a.e1 = a.ev * k.i1;
-    const float m2 = tme * k.i2;
+    const float m2 = ds2 * r;  // (tau - ev) / sigT^2
     const float reuse = (t2 - 2.f * tau) + a.e1 * k.i2x2;
     if constexpr (HAS_C2) {
-        const float p6 = (t2 * t2) * r4;
         const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
         const float q1m = k.a1 + k.b1 * zin;
-        a.fpart = q0 * m2 + q1m * reuse + k.kfi6 * p6;
+        a.fpart = q0 * m2 + q1m * reuse + (k.q2m2 * ds4) * r4;
         a.src = q0 * a.e1 + q1m * m2 + k.q2m * reuse;
     } else {
         const float q0 = k.c0 + k.c1 * zin;
@@ -670,6 +673,9 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     auto series_ok = [&](const RawSeg &sg) {
                         return EXPM == SMOC_EXP_ACCURATE && __float_as_uint(sg.len) < thr;
                     };
+                    constexpr bool kBlockSum = MODE == MODE_SWEEP &&
+                                               (TALLY == SMOC_TALLY_DET_F64 ||
+                                                TALLY == SMOC_TALLY_ATOMIC_F64);
                     auto tally = [&](int j, float fi, float ev) {
                         if constexpr (MODE == MODE_XFER)
                             mprod[j] *= 1.f - ev;
@@ -720,15 +726,29 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                 for (int u = 0; u < N; u++) {
                                     const RawSeg &sg = u == 0 ? cur : nx[u - 1];
                                     const float z2 = HAS_C2 ? sg.zin * sg.zin : 0.f;
+                                    const float d2 = sg.len * sg.len;
+                                    const float d4 = HAS_C2 ? d2 * d2 : 0.f;
 #pragma unroll
                                     for (int j = 0; j < GPL; j++)
-                                        a[u][j] = series_a<HAS_C2>(rk[j], sg.len, sg.zin, z2);
+                                        a[u][j] = series_a<HAS_C2>(rk[j], sg.len, d2, d4, sg.zin,
+                                                                   z2);
                                 }
 #pragma unroll
-                                for (int j = 0; j < GPL; j++)
+                                for (int j = 0; j < GPL; j++) {
+                                    if constexpr (kBlockSum) {
+                                        // the block's flux integrals summed in
+                                        // FP32, then once into the FP64 run sum
+                                        float bs = series_b(a[0][j], psi[j]);
 #pragma unroll
-                                    for (int u = 0; u < N; u++)
-                                        tally(j, series_b(a[u][j], psi[j]), a[u][j].ev);
+                                        for (int u = 1; u < N; u++)
+                                            bs += series_b(a[u][j], psi[j]);
+                                        acc[j].add(bs);
+                                    } else {
+#pragma unroll
+                                        for (int u = 0; u < N; u++)
+                                            tally(j, series_b(a[u][j], psi[j]), a[u][j].ev);
+                                    }
+                                }
                                 k += N;
                                 cur = nx[N - 1];
                             };
