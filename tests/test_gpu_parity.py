@@ -389,3 +389,65 @@ def test_segment_count_limit(gpu_ready):
                                           off, np.ascontiguousarray(prob.psi_in[:1]).ravel())
         assert rc == smoc.ERR_ARG
         assert "2^31" in smoc.lib().smoc_last_error().decode()
+
+
+def _block_boundary_stream(seed, G=100, R=40, F=5, T=90):
+    """A hand-built region-sorted stream for the run loop's block logic: runs
+    of 1-19 segments, so runs end at every offset of an 8-segment block;
+    tracks of 1-45 segments, so items end mid-block with the next track often
+    continuing the same run; lengths around the series threshold (0.5 x to
+    3 x of 1 / max sigT), exact zeros and negative zeros."""
+    rng = np.random.default_rng(seed)
+    sig = rng.uniform(0.1, 10.0, (R, G)).astype(np.float32)
+    thr = (1.0 / sig.max(axis=1)).astype(np.float32)
+    lens, regs, zs, off = [], [], [], [0]
+    run_left, r, f = 0, 0, 0
+    for _ in range(T):
+        n = int(rng.integers(1, 46))
+        for _ in range(n):
+            if run_left == 0:
+                run_left = int(rng.integers(1, 20))
+                r, f = int(rng.integers(R)), int(rng.integers(F))
+            run_left -= 1
+            u = rng.random()
+            if u < 0.03:
+                ln = 0.0
+            elif u < 0.06:
+                ln = -0.0
+            elif u < 0.2:
+                ln = thr[r] * rng.uniform(1.0, 3.0)  # above the threshold: direct exp
+            else:
+                ln = thr[r] * rng.uniform(0.0, 0.999)
+            lens.append(ln)
+            regs.append(r)
+            zs.append((f + rng.uniform(0.05, 0.95)) / F)
+        off.append(off[-1] + n)
+    S = off[-1]
+    prob = smoc.Problem(G, R, F, sig, rng.random((R, F, G), dtype=np.float32),
+                        np.array(lens, np.float32), np.array(regs, np.int32),
+                        np.array(zs, np.float32), np.array(off, np.int64),
+                        rng.random((T, G), dtype=np.float32))
+    same = (prob.region[1:] == prob.region[:-1]).mean()
+    assert same > 0.5 and S > 1000  # the run loop is the one taken
+    assert np.signbit(prob.len).any() and (prob.len == 0).any()
+    return prob
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_sorted_block_boundaries(gpu_ready, seed):
+    """Runs and items ending at every offset of a block, lengths on both
+    sides of the series threshold, zero and -0.0 lengths: the run loop
+    against the FP64 oracle, and the deterministic tally bit-exact across
+    launch shapes."""
+    prob = _block_boundary_stream(seed)
+    print(f"case sorted block boundaries seed {seed}: S {prob.S}", flush=True)
+    for tally in ("det-f64", "atomic-f64", "atomic-f32", "fixed64"):
+        flux, psi, _ = run_gpu(prob, "f32", "accurate", tally)
+        check_vs_oracle(prob, flux, psi)
+    ref = None
+    for gpl, wpb in ((1, 1), (2, 4), (2, 16), (1, 8)):
+        out = run_gpu(prob, "f32", "accurate", "det-f64", groups_per_lane=gpl,
+                      waves_per_block=wpb)[:2]
+        if ref is None:
+            ref = out
+        assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (gpl, wpb)
