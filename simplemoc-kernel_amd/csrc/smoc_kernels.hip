@@ -635,8 +635,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         // derives the run constants and, at the end, flushes the run's tally
         // summed in registers.  The inner loop walks the run in blocks: the
         // stream entries of the next kSeries segments are loaded ahead (the
-        // device arrays are padded, so loads past the item's end are in bounds
-        // and only marked past-the-end), and up to kSeries consecutive
+        // device arrays are padded, so loads past the item's end are in
+        // bounds; blocks never reach past it), and up to kSeries consecutive
         // segments that all take the series branch have their psi-free parts
         // evaluated together before their psi updates run in order.
         if (kb32 < ke32) {
