@@ -361,6 +361,27 @@ __device__ __forceinline__ void acc_init(Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, f
     acc.lim = lim / SMOC_WEIGHT;
 }
 
+// Block sums (region-sorted stream, FP64 and FIXED64 tallies): each segment's
+// flux integral is checked as a contribution (FIXED64's bound, nothing for
+// the others), and the block's FP32 sum joins the run sum with one add.
+template <typename A> __device__ __forceinline__ bool acc_fits(const A &, float) { return true; }
+template <bool SINGLE>
+__device__ __forceinline__ bool acc_fits(const Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, float x)
+{
+    return fabsf(x) <= acc.lim;
+}
+template <typename A> __device__ __forceinline__ void acc_add_block(A &acc, float sum, bool)
+{
+    acc.add(sum);
+}
+template <bool SINGLE>
+__device__ __forceinline__ void acc_add_block(Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, float sum,
+                                              bool fits)
+{
+    acc.bad |= !fits;
+    acc.s += (double)sum;
+}
+
 template <int TALLY> struct AccSingle {
     float s = 0.f;
     __device__ __forceinline__ void add(float x) { s = x; }
@@ -673,9 +694,14 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     auto series_ok = [&](const RawSeg &sg) {
                         return EXPM == SMOC_EXP_ACCURATE && __float_as_uint(sg.len) < thr;
                     };
+                    // blocks of up to NB segments: 8, or 4 where blocks of 8
+                    // would need more than the 128 VGPRs a wave may hold at
+                    // 16 waves per workgroup (FIXED64 with 2 groups per lane)
+                    constexpr int NB = TALLY == SMOC_TALLY_FIXED64 && GPL == 2 ? 4 : kSeries;
                     constexpr bool kBlockSum = MODE == MODE_SWEEP &&
                                                (TALLY == SMOC_TALLY_DET_F64 ||
-                                                TALLY == SMOC_TALLY_ATOMIC_F64);
+                                                TALLY == SMOC_TALLY_ATOMIC_F64 ||
+                                                TALLY == SMOC_TALLY_FIXED64);
                     auto tally = [&](int j, float fi, float ev) {
                         if constexpr (MODE == MODE_XFER)
                             mprod[j] *= 1.f - ev;
@@ -691,12 +717,12 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                         const float *__restrict__ lp = len + (size_t)k + 1;
                         const int32_t *__restrict__ cp = code + (size_t)k + 1;
                         const float *__restrict__ zp = zin + (size_t)k + 1;
-                        RawSeg nx[kSeries];
+                        RawSeg nx[NB];
 #pragma unroll
-                        for (int u = 0; u < kSeries; u++)
+                        for (int u = 0; u < NB; u++)
                             nx[u] = RawSeg{lp[u], zp[u], cp[u]};
                         if (series_ok(cur)) {
-                            // up to kSeries segments of the run at once when
+                            // up to NB segments of the run at once when
                             // all take the series: their psi-free parts are
                             // independent of each other.  Common case first,
                             // in a few scalar operations: every look-ahead
@@ -707,14 +733,14 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             // bits are 2^31, sets it as series_ok() rejects it
                             uint32_t diff = 0, below = 0;
 #pragma unroll
-                            for (int u = 0; u < kSeries - 1; u++) {
+                            for (int u = 0; u < NB - 1; u++) {
                                 diff |= (uint32_t)(nx[u].code ^ ccode);
                                 below |= thr - 1u - __float_as_uint(nx[u].len);
                             }
-                            uint32_t n = kSeries;
+                            uint32_t n = NB;
                             if ((diff | (below & 0x80000000u)) != 0) {
                                 n = 1;
-                                while (n < kSeries && nx[n - 1].code == ccode &&
+                                while (n < NB && nx[n - 1].code == ccode &&
                                        series_ok(nx[n - 1]))
                                     ++n;
                             }
@@ -739,10 +765,14 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                         // the block's flux integrals summed in
                                         // FP32, then once into the FP64 run sum
                                         float bs = series_b(a[0][j], psi[j]);
+                                        bool fits = acc_fits(acc[j], bs);
 #pragma unroll
-                                        for (int u = 1; u < N; u++)
-                                            bs += series_b(a[u][j], psi[j]);
-                                        acc[j].add(bs);
+                                        for (int u = 1; u < N; u++) {
+                                            const float fi = series_b(a[u][j], psi[j]);
+                                            fits &= acc_fits(acc[j], fi);
+                                            bs += fi;
+                                        }
+                                        acc_add_block(acc[j], bs, fits);
                                     } else {
 #pragma unroll
                                         for (int u = 0; u < N; u++)
@@ -752,11 +782,11 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                 k += N;
                                 cur = nx[N - 1];
                             };
-                            // the largest power of two <= n (and <= kSeries)
-                            if (kSeries >= 8 && n >= 8)
-                                block(std::integral_constant<int, (kSeries >= 8 ? 8 : 1)>{});
-                            else if (kSeries >= 4 && n >= 4)
-                                block(std::integral_constant<int, (kSeries >= 4 ? 4 : 1)>{});
+                            // the largest power of two <= n (and <= NB)
+                            if (NB >= 8 && n >= 8)
+                                block(std::integral_constant<int, (NB >= 8 ? 8 : 1)>{});
+                            else if (NB >= 4 && n >= 4)
+                                block(std::integral_constant<int, (NB >= 4 ? 4 : 1)>{});
                             else if (n >= 2)
                                 block(std::integral_constant<int, 2>{});
                             else
