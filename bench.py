@@ -23,6 +23,10 @@ Also measured, same shape:
     `atomic_tally_floor` times the same stream's per-segment FP32 atomic
     flushes alone (k_tally_probe: no gathers, no arithmetic), the part of
     the sweep that no data layout removes;
+  * `mixed.roofline`: the sorted sweep is bound by VALU issue, so its figure
+    is VALU lane-operations per second (the committed profile's
+    SQ_INSTS_VALU per launch x 64 over the step time) against the FP32 issue
+    ceiling of 7.86e13 (157.3 TFLOP/s of FMAs);
   * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
     the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
@@ -45,6 +49,10 @@ import smoc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_plain_50Mx100_traffic.txt")
+MIXED_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_mixed_sorted_bf16_50Mx100.txt")
+# FP32 VALU issue ceiling: 157.3 TFLOP/s = 7.86e13 FMA lane-operations/s, which gfx950 reaches by
+# issuing two VALU instructions per SIMD per quad-cycle (packed FP32 is not used here)
+VALU_PEAK_LANE_OPS = 157.3e12 / 2
 
 
 def alg_bytes_per_sg(storage: str, tally: str, G: int, L: float) -> float:
@@ -103,6 +111,17 @@ def measured_traffic():
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def mixed_valu_instructions():
+    """VALU instructions per k_sweep launch of the mixed workload from the
+    committed rocprofv3 summary (SQ_INSTS_VALU; fixed for a given input)."""
+    try:
+        txt = open(MIXED_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"k_sweep\s+SQ_INSTS_VALU\s+([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
 
 
@@ -307,6 +326,18 @@ def main():
             "hbm_gbs_alg": mB * sg / (mtotal * 1e-3) / 1e9, "alg_bytes_per_sg": mB,
             "launch": mused,
             "checksums": dict(zip(("fine_flux", "state_flux"), smoc.checksums(mf, mp)))}
+        # (the profile's launch has 2 groups per lane; the count holds for that shape)
+        valu = mixed_valu_instructions() if ((S, G, L) == (50_000_000, 100, 100) and
+                                             mused.get("groups_per_lane") == 2) else None
+        if valu is not None:
+            # lane-operations per second over the whole step (reset + sweep + reduce): a
+            # lower bound for k_sweep's own rate
+            ach = valu * 64 / (mwall / args.steps)
+            out["mixed"]["roofline"] = {
+                "bound": "valu", "achieved": ach, "peak": VALU_PEAK_LANE_OPS,
+                "unit": "VALU lane-ops/s", "frac": ach / VALU_PEAK_LANE_OPS,
+                "kernel": "k_sweep", "valu_instructions_per_launch": valu,
+                "source": os.path.relpath(MIXED_SUMMARY, ROOT)}
         # the same sorted path with FP32 storage: no quantisation, so its error
         # against FP64 on the FP32 masters is the arithmetic's alone
         with smoc.Context(G, R, F, storage="f32", exp=args.exp, tally=mtally) as ctx:
