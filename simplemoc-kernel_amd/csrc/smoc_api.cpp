@@ -1044,7 +1044,7 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
         if (rc != SMOC_OK)
             return rc;
         double best = 1e30;
-        for (int r = 0; r < 2; r++) {
+        for (int r = 0; r < 3; r++) {
             rc = run_once(c, l, &t);
             if (rc != SMOC_OK)
                 return rc;
@@ -1058,6 +1058,14 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
     best.chunk = 0;
     best.persistent = 0;
     double best_ms = 0.0;
+    // a few runs first, so that the baseline is not timed on a GPU still
+    // raising its clocks (a candidate measured later would win on that alone)
+    for (int r = 0; r < 3; r++) {
+        smoc_timing t{};
+        int rc = run_once(c, best, &t);
+        if (rc != SMOC_OK)
+            return rc;
+    }
     int rc = measure(best, &best_ms);
     if (rc != SMOC_OK)
         return rc;
@@ -1071,7 +1079,7 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
             return SMOC_OK;
         if (r != SMOC_OK)
             return r;
-        if (ms < best_ms) {
+        if (ms < 0.99 * best_ms) {  // a change must gain more than timing noise
             best_ms = ms;
             best = cand;
         }
