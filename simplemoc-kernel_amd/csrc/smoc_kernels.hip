@@ -477,8 +477,8 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 // (ACCURATE exp, checked once per segment on the scalar unit): the series
 // alone, with tau^2 shared by the series and reuse, and poly / 6 = tau^4 r4.
 // Split in two: series_a holds everything that does not depend on psi, so a
-// wave can evaluate it for two segments before the psi updates run one after
-// the other (series_b).  HAS_C2 = false: a first or last fine interval, whose
+// wave can evaluate it for a block of segments before the psi updates run one
+// after the other (series_b).  HAS_C2 = false: a first or last fine interval, whose
 // fit is linear (c2 = 0, so q2 = 0 and q1 = c1): the terms in c2 are exactly
 // zero and skipped, with the same results.
 struct SegA {
