@@ -837,9 +837,11 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     const int slices = (G + 64 * gpl - 1) / (64 * gpl);
     const int64_t nrow = c->n_fsr();
     const bool chunked = l.chunk > 0;
-    if (chunked && c->det() && c->runs) {
+    // FIXED64 too: a chunk boundary would split a run's sum, and the run sums
+    // are what is rounded to fixed point, so results would depend on the chunk
+    if (chunked && (c->det() || c->fixed()) && c->runs) {
         smoc_set_error("smoc_run: chunked sweeps of a region-sorted stream need an atomic "
-                       "tally (deterministic runs would cross chunk boundaries)");
+                       "FP32/FP64 tally (deterministic runs would cross chunk boundaries)");
         return SMOC_ERR_ARG;
     }
     if (chunked && c->plan.chunk != l.chunk) {

@@ -694,10 +694,12 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     auto series_ok = [&](const RawSeg &sg) {
                         return EXPM == SMOC_EXP_ACCURATE && __float_as_uint(sg.len) < thr;
                     };
-                    // blocks of up to NB segments: 8, or 4 where blocks of 8
-                    // would need more than the 128 VGPRs a wave may hold at
-                    // 16 waves per workgroup (FIXED64 with 2 groups per lane)
-                    constexpr int NB = TALLY == SMOC_TALLY_FIXED64 && GPL == 2 ? 4 : kSeries;
+                    // blocks of up to NB segments: 8, or 4 for FIXED64, whose
+                    // blocks of 8 need more than the 128 VGPRs a wave may hold
+                    // at 16 waves per workgroup with 2 groups per lane (the
+                    // same 4 at 1 group per lane keeps the block sums, and so
+                    // the results, identical across launch shapes)
+                    constexpr int NB = TALLY == SMOC_TALLY_FIXED64 ? 4 : kSeries;
                     constexpr bool kBlockSum = MODE == MODE_SWEEP &&
                                                (TALLY == SMOC_TALLY_DET_F64 ||
                                                 TALLY == SMOC_TALLY_ATOMIC_F64 ||

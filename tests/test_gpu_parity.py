@@ -444,10 +444,18 @@ def test_sorted_block_boundaries(gpu_ready, seed):
     for tally in ("det-f64", "atomic-f64", "atomic-f32", "fixed64"):
         flux, psi, _ = run_gpu(prob, "f32", "accurate", tally)
         check_vs_oracle(prob, flux, psi)
-    ref = None
-    for gpl, wpb in ((1, 1), (2, 4), (2, 16), (1, 8)):
-        out = run_gpu(prob, "f32", "accurate", "det-f64", groups_per_lane=gpl,
-                      waves_per_block=wpb)[:2]
-        if ref is None:
-            ref = out
-        assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), (gpl, wpb)
+    for tally in ("det-f64", "fixed64"):
+        ref = None
+        for gpl, wpb in ((1, 1), (2, 4), (2, 16), (1, 8)):
+            out = run_gpu(prob, "f32", "accurate", tally, groups_per_lane=gpl,
+                          waves_per_block=wpb)[:2]
+            if ref is None:
+                ref = out
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), \
+                (tally, gpl, wpb)
+    # chunking would split runs: refused for the tallies that promise bit-exact results
+    with smoc.Context(prob.G, prob.R, prob.F, tally="fixed64") as ctx:
+        ctx.load(prob)
+        with pytest.raises(smoc.SmocError) as e:
+            ctx.run(chunk=16)
+        assert e.value.code == smoc.ERR_ARG
