@@ -69,16 +69,21 @@ extern "C" {
  *               src/cuda/kernel.cu:181); order-dependent rounding.
  *   ATOMIC_F64  FP64 device atomics into an FP64 fine_flux.
  *   DET_F64     deterministic: each run of consecutive segments of a track
- *               in one (region, interval) is summed in FP64 registers and
- *               stored; a segmented reduction then sums the runs of every
- *               (region, interval) in a fixed order.  Bit-exact run to run.
+ *               in one (region, interval) is summed in FP64 registers (on a
+ *               region-sorted stream, FP32 sums of blocks of up to 8
+ *               segments join the FP64 run sum) and stored; a segmented
+ *               reduction then sums the runs of every (region, interval) in
+ *               a fixed order.  Bit-exact run to run and across launch
+ *               shapes; chunked sweeps of a region-sorted stream are refused
+ *               (SMOC_ERR_ARG), since chunks would split runs.
  *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums.
  *   FIXED64     deterministic: each run is rounded to a 64-bit fixed-point
  *               integer (2^-32 resolution, |fine_flux| < 2^31) and added with
  *               integer atomics, which are associative, so the result is
  *               bit-exact run to run in any order; converted to FP64 after
  *               the sweep.  smoc_run fails with SMOC_ERR_RANGE if a value
- *               left the range. */
+ *               left the range.  Chunked sweeps of a region-sorted stream
+ *               are refused as for DET_F64. */
 #define SMOC_TALLY_ATOMIC_F32 0
 #define SMOC_TALLY_ATOMIC_F64 1
 #define SMOC_TALLY_DET_F64 2
