@@ -698,8 +698,11 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     // blocks of 8 need more than the 128 VGPRs a wave may hold
                     // at 16 waves per workgroup with 2 groups per lane (the
                     // same 4 at 1 group per lane keeps the block sums, and so
-                    // the results, identical across launch shapes)
-                    constexpr int NB = TALLY == SMOC_TALLY_FIXED64 ? 4 : kSeries;
+                    // the results, identical across launch shapes); 1 where
+                    // the exp mode has no series branch (one segment a step)
+                    constexpr int NB = EXPM != SMOC_EXP_ACCURATE          ? 1
+                                       : TALLY == SMOC_TALLY_FIXED64 ? 4
+                                                                     : kSeries;
                     constexpr bool kBlockSum = MODE == MODE_SWEEP &&
                                                (TALLY == SMOC_TALLY_DET_F64 ||
                                                 TALLY == SMOC_TALLY_ATOMIC_F64 ||
@@ -711,8 +714,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             acc[j].add(fi);
                     };
                     for (;;) {
-                        // the next kSeries segments, from one 64-bit base per
-                        // array so that the scalar loads take constant offsets
+                        // the next NB segments, from one 64-bit base per array
+                        // so that the scalar loads take constant offsets
                         // (the arrays are padded, so the loads past the
                         // item's end are in bounds; such entries are never
                         // used: blocks are capped by the segments left)
