@@ -117,6 +117,9 @@ typedef struct {
     int persistent;      /* 1: persistent workgroups pulling work from a queue */
     int use_graph;       /* 1: capture the run into a hipGraph, replay it      */
     int autotune;        /* 1: measure candidates on first run, keep the best  */
+    int reduce_waves;    /* deterministic tallies: wavefronts per workgroup of
+                            the segmented reduction (1..16; default 4); the
+                            summation order does not depend on it           */
 } smoc_launch;
 
 typedef struct {

@@ -754,6 +754,8 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
         d.chunk = 0;  // nothing to split
     d.persistent = d.persistent ? 1 : 0;
     d.use_graph = d.use_graph ? 1 : 0;
+    if (d.reduce_waves < 1 || d.reduce_waves > 16)
+        d.reduce_waves = 4;
     *l = d;
 }
 
@@ -949,7 +951,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
         r.Gp = c->Gp;
         r.slices = slices;
         r.single = c->runs ? 0 : 1;
-        LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, 4, c->stream));
+        LAUNCHCHK(smoc_launch_reduce(&r, c->p.tally, gpl, l.reduce_waves, c->stream));
     }
     if (c->fixed())
         LAUNCHCHK(smoc_launch_fixed_to_f64(c->fixed_flux(), c->flux_init.p, nrow * c->Gp,
@@ -962,7 +964,8 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
 static bool same_launch(const smoc_launch &x, const smoc_launch &y)
 {
     return x.groups_per_lane == y.groups_per_lane && x.waves_per_block == y.waves_per_block &&
-           x.chunk == y.chunk && x.persistent == y.persistent;
+           x.chunk == y.chunk && x.persistent == y.persistent &&
+           x.reduce_waves == y.reduce_waves;
 }
 
 // One run, timed with events on the context stream; with use_graph the
@@ -1118,6 +1121,14 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
         cand.waves_per_block = w;
         if ((rc = consider(cand)) != SMOC_OK)
             return rc;
+    }
+    if (c->det()) {  // the reduction's tile: waves per workgroup
+        for (int w : {1, 2, 8, 16}) {
+            smoc_launch cand = best;
+            cand.reduce_waves = w;
+            if ((rc = consider(cand)) != SMOC_OK)
+                return rc;
+        }
     }
     best.use_graph = base.use_graph;
     *best_out = best;

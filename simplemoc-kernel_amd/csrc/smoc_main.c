@@ -412,9 +412,9 @@ int main(int argc, char *argv[])
     printf("%-25s%.4e\n", "Attenuations/s:", (double)S * G / mean_s);
     printf("%-25s%.3f ms (transfer+scan %.3f, sweep %.3f, reduce %.3f)\n", "Best run:",
            best.total_ms, best.xfer_ms, best.sweep_ms, best.reduce_ms);
-    printf("%-25sgpl %d, wpb %d, chunk %d, persistent %d, graph %d\n", "Launch:",
+    printf("%-25sgpl %d, wpb %d, chunk %d, persistent %d, graph %d, reduce wpb %d\n", "Launch:",
            best.used.groups_per_lane, best.used.waves_per_block, best.used.chunk,
-           best.used.persistent, best.used.use_graph);
+           best.used.persistent, best.used.use_graph, best.used.reduce_waves);
     printf("%-25s%.17g\n", "fine_flux checksum:", ck_flux);
     printf("%-25s%.17g\n", "state_flux checksum:", ck_psi);
     border_print();

@@ -48,7 +48,7 @@ class Launch(ctypes.Structure):
     _fields_ = [("seg_per_block", ctypes.c_int), ("groups_per_lane", ctypes.c_int),
                 ("waves_per_block", ctypes.c_int), ("chunk", ctypes.c_int),
                 ("persistent", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("autotune", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("reduce_waves", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

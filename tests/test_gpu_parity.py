@@ -446,13 +446,13 @@ def test_sorted_block_boundaries(gpu_ready, seed):
         check_vs_oracle(prob, flux, psi)
     for tally in ("det-f64", "fixed64"):
         ref = None
-        for gpl, wpb in ((1, 1), (2, 4), (2, 16), (1, 8)):
+        for gpl, wpb, rw in ((1, 1, 1), (2, 4, 4), (2, 16, 16), (1, 8, 2)):
             out = run_gpu(prob, "f32", "accurate", tally, groups_per_lane=gpl,
-                          waves_per_block=wpb)[:2]
+                          waves_per_block=wpb, reduce_waves=rw)[:2]
             if ref is None:
                 ref = out
             assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), \
-                (tally, gpl, wpb)
+                (tally, gpl, wpb, rw)
     # chunking would split runs: refused for the tallies that promise bit-exact results
     with smoc.Context(prob.G, prob.R, prob.F, tally="fixed64") as ctx:
         ctx.load(prob)
