@@ -887,6 +887,12 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
             g = std::min<int64_t>(g, resident_blocks);
         return g;
     };
+    // persistent work batches: about 1/16 of a wave's fair share per counter
+    // update (1 for the few long chunks of a batched run), at most 64
+    auto batch_for = [&](int64_t n_entries) {
+        const int64_t fair = n_entries * slices / std::max<int64_t>(1, resident_blocks * wpb);
+        return (int)std::min<int64_t>(64, std::max<int64_t>(1, fair / 16));
+    };
 
     if (ev)
         HIPCHK(hipEventRecord(ev[0], c->stream));
@@ -910,6 +916,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
         p1.n_items = pl.n1;
         if (l.persistent) {
             p1.queue = c->queue.p;
+            p1.queue_batch = batch_for(pl.n1);
             LAUNCHCHK(smoc_launch_zero_i64(c->queue.p, 1, c->stream));
         }
         LAUNCHCHK(smoc_launch_sweep(&p1, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
@@ -933,6 +940,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     if (a.n_items > 0) {
         if (l.persistent) {
             a.queue = c->queue.p + 1;
+            a.queue_batch = batch_for(a.n_items);
             LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
         }
         LAUNCHCHK(smoc_launch_sweep(&a, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,

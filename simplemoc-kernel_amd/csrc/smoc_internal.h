@@ -52,6 +52,8 @@ typedef struct {
     int64_t n_items;             /* work entries: tracks, chunks or list length */
     int xfer_mode;               /* 1: phase-1 launch, mode chosen per entry */
     int64_t *queue;              /* persistent mode: work counter, or NULL    */
+    int queue_batch;             /* persistent mode: consecutive work items a
+                                    wave takes per counter update (>= 1)     */
     float tab_dx, tab_max;
     int tab_n;
     int G, Gp, F, slices;        /* Gp: padded row stride (multiple of 128) */
@@ -116,7 +118,7 @@ int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *str
 int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);
-// k_tally_probe over S segments into [rows][Gp] FP32; grid = ceil(ceil(S / 100) / 4).
+/* k_tally_probe over S segments into [rows][Gp] FP32; grid = ceil(ceil(S / 100) / 4) */
 int smoc_launch_tally_probe(void *flux, int64_t S, int rows, int G, int Gp, int grid,
                             void *stream);
 /* bytes per packed tuple for a storage type */

@@ -916,15 +916,23 @@ __global__ void __launch_bounds__(1024)
     const ExpTab tb = stage_table<EXPM>(a, s_tab);
     const int lane = threadIdx.x & 63;
     const int64_t n_work = a.n_items * a.slices;
+    // a wave takes queue_batch consecutive items per counter update: with
+    // millions of short items one counter updated per item serialises the
+    // grid on that address; the counter only grows, so every wave reaches
+    // first >= n_work and exits
+    const int64_t batch = a.queue_batch > 0 ? a.queue_batch : 1;
     for (;;) {
         unsigned long long got = 0;
         if (lane == 0)
-            got = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue), 1ull);
-        const int64_t item = uniform64((int64_t)__shfl(got, 0));
-        if (item >= n_work)
+            got = atomicAdd(reinterpret_cast<unsigned long long *>(a.queue),
+                            (unsigned long long)batch);
+        const int64_t first = uniform64((int64_t)__shfl(got, 0));
+        if (first >= n_work)
             break;
-        dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off, items,
-                                                           item, lane, tb);
+        const int64_t last = first + batch < n_work ? first + batch : n_work;
+        for (int64_t item = first; item < last; ++item)
+            dispatch_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off,
+                                                               items, item, lane, tb);
     }
 }
 
