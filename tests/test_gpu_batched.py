@@ -135,3 +135,26 @@ def test_fixed64_chunked_graph(gpu_ready, ragged, ragged_oracle, storage):
     of, op = (ragged_oracle if storage == "f32" else
               pyoracle.sweep_f64(quantized(ragged, storage), pyoracle.EXP_NATIVE, nthreads=8))
     assert normwise(f1, of) < 1e-5 and normwise(p1, op) < 1e-5
+
+
+@pytest.mark.parametrize("sorted_stream", [False, True])
+def test_persistent_batched_pulls(gpu_ready, sorted_stream):
+    """Enough short tracks that a persistent wave takes several consecutive
+    items per update of the work counter (400,000 items over at most 8192
+    resident waves at 16 groups: batches of 3): the deterministic tally is
+    bit-identical to the grid launch's, and both match the FP64 oracle."""
+    prob = smoc.generate(1_200_000, G=16, R=2000, F=5, seed=23, track_len=3,
+                         sorted_stream=sorted_stream)
+    assert prob.T == 400_000
+    print(f"case persistent batched pulls, sorted {sorted_stream}", flush=True)
+    out = []
+    with smoc.Context(16, 2000, 5, tally="det-f64") as ctx:
+        ctx.load(prob)
+        for pers in (0, 1):
+            ctx.run(persistent=pers)
+            out.append(ctx.results())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=8)
+    e = (normwise(out[1][0], of), normwise(out[1][1], op))
+    print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+    assert max(e) < 1e-5
