@@ -580,8 +580,13 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
     // the loop bounds are 32-bit scalar compares; k + kSeries cannot wrap
     // unsigned
     const uint32_t kb32 = (uint32_t)kb, ke32 = (uint32_t)ke;
+    // the stream entry of segment k, marked past-the-end (code -1) beyond the
+    // item: loaded unconditionally (the arrays are padded and k <= S + 1), so
+    // only the mark is a select
     auto raw_at = [&](uint32_t k) {
-        return k < ke32 ? load_raw(len, code, zin, k) : RawSeg{0.f, 0.f, -1};
+        RawSeg r = load_raw(len, code, zin, (int64_t)k);
+        r.code = k < ke32 ? r.code : -1;
+        return r;
     };
     auto flush_all = [&](int row) {
 #pragma unroll
