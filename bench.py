@@ -8,14 +8,27 @@ plain kernel at the reference's default scale.  A step is one smoc_run over
 the whole stream (fine_flux reset + sweep); inputs are resident in HBM
 before timing starts.  `value` is segment-group attenuations/s of that step.
 
-Also measured, same shape:
+Also measured, same shape (each line its own context, autotuned where it
+says so, timed over the same number of steps):
   * `mixed`: the batched mixed-precision path -- the stream pre-sorted by
-    (region, fine interval), BF16 storage of sigT / fine_source, FP32 math,
-    runs summed in registers and a deterministic FP64 segmented reduction,
-    launched as the autotuner picks it (during the warm-up steps) and replayed
-    from a captured hipGraph; `mixed_f32_storage` is the same path with FP32
-    storage, whose error against FP64 on the FP32 masters is the arithmetic's
-    alone;
+    (region, fine interval) and cut into tracks of 100 consecutive sorted
+    segments, FP32 storage, FP32 math, runs of a track in one (region,
+    interval) summed in registers, deterministic FP64 segmented reduction,
+    launched as the autotuner picks it (during the warm-up steps) and
+    replayed from a captured hipGraph; it meets the 1e-5 bar against FP64 on
+    the FP32 masters.  `mixed_bf16_storage` / `mixed_f16_storage` are the
+    same path with 16-bit sigT / fine_source (their errors against both
+    oracle definitions are in `accuracy`);
+  * attribution of the mixed-over-plain ratio, each on one change at a time:
+    `plain_sorted_segment_flush` is the plain kernel (FP32, per-segment FP32
+    atomics) on the sorted stream -- the stream order alone;
+    `plain_sorted` adds the run sums (one flush per run, atomic FP32);
+    `mixed` then swaps in the deterministic FP64 tally, autotuning and the
+    graph; `mixed_track_order` is the mixed configuration on the plain
+    problem's own track-order stream (every run one segment: per-segment FP32
+    partials and the fixed-order reduction) -- the mixed kernel without
+    sorting.  `sort` is the host time of the stable counting sort that
+    produced the sorted stream, outside every timed region;
   * `roofline`: algorithmic bytes per segment-group of the dominant kernel
     (k_sweep) over its mean duration from HIP events on the library stream;
     `traffic` comes from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE
@@ -26,12 +39,13 @@ Also measured, same shape:
   * `mixed.roofline`: the sorted sweep is bound by VALU issue, so its figure
     is VALU lane-operations per second (the committed profile's
     SQ_INSTS_VALU per launch x 64 over the step time) against the FP32 issue
-    ceiling of 7.86e13 (157.3 TFLOP/s of FMAs);
+    ceiling of 7.86e13 (157.3 TFLOP/s of FMAs), with `useful_frac` counting
+    only the lanes that carry one of the G groups;
   * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
     the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
     on this host's cores on a bounded sample of the same stream, plus the
-    accuracy of both GPU paths against the FP64 oracle (the checker).
+    accuracy of the GPU paths against the FP64 oracle (the checker).
 """
 import argparse
 import json
@@ -49,7 +63,7 @@ import smoc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_plain_50Mx100_traffic.txt")
-MIXED_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_mixed_sorted_bf16_50Mx100.txt")
+MIXED_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_mixed_sorted_f32_50Mx100.txt")
 # FP32 VALU issue ceiling: 157.3 TFLOP/s = 7.86e13 FMA lane-operations/s, which gfx950 reaches by
 # issuing two VALU instructions per SIMD per quad-cycle (packed FP32 is not used here)
 VALU_PEAK_LANE_OPS = 157.3e12 / 2
@@ -61,7 +75,7 @@ def alg_bytes_per_sg(storage: str, tally: str, G: int, L: float) -> float:
     shared by G groups) + psi in and out (8 B per track-group, shared by the
     L segments of a track)."""
     b = 4 if storage == "f32" else 2
-    bt = {"atomic-f32": 4, "atomic-f64": 8, "det-f64": 8, "det-kahan": 8}[tally]
+    bt = {"atomic-f32": 4, "atomic-f64": 8, "det-f64": 8, "det-kahan": 8, "fixed64": 8}[tally]
     return b + 3 * b + 2 * bt + 12.0 / G + 8.0 / L
 
 
@@ -73,9 +87,9 @@ def parse():
     ap.add_argument("--segments", type=int, default=50_000_000)
     ap.add_argument("--groups", type=int, default=100)
     ap.add_argument("--track-len", type=int, default=100)
-    ap.add_argument("--storage", default="f32")
-    ap.add_argument("--exp", default="accurate")
-    ap.add_argument("--tally", default="atomic-f32")
+    ap.add_argument("--storage", default="f32", choices=sorted(smoc.STORAGE))
+    ap.add_argument("--exp", default="accurate", choices=sorted(smoc.EXP))
+    ap.add_argument("--tally", default="atomic-f32", choices=sorted(smoc.TALLY))
     ap.add_argument("--gpl", type=int, default=0)
     ap.add_argument("--wpb", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
@@ -189,32 +203,33 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
     if accuracy:
         acc = {}
         t0 = time.time()
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from conftest import quantized
+
+        def errs(f, p, of, op):
+            return {"fine_flux_checksum_rel": abs(f.sum() - of.sum()) / abs(of.sum()),
+                    "state_flux_checksum_rel":
+                        abs(p.astype(np.float64).sum() - op.sum()) / abs(op.sum()),
+                    "fine_flux_normwise": float(np.linalg.norm(f - of) / np.linalg.norm(of)),
+                    "state_flux_normwise": float(np.linalg.norm(p.astype(np.float64) - op) /
+                                                 np.linalg.norm(op))}
+
         of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=threads)
         f, p = gpu_plain
-        acc["plain_vs_fp64"] = {
-            "fine_flux_checksum_rel": abs(f.sum() - of.sum()) / abs(of.sum()),
-            "state_flux_checksum_rel": abs(p.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+        acc["plain_vs_fp64"] = errs(f, p, of, op)
         if mixed is not None:
-            mprob, mf, mp, storage, f32_run = mixed
+            if mixed.get("track_order") is not None:
+                f, p = mixed["track_order"]
+                acc["mixed_track_order_vs_fp64"] = errs(f, p, of, op)
+            mprob = mixed["prob"]
             of, op = pyoracle.sweep_f64(mprob, pyoracle.EXP_NATIVE, nthreads=threads)
-            acc["mixed_vs_fp64_on_fp32_masters"] = {
-                "fine_flux_checksum_rel": abs(mf.sum() - of.sum()) / abs(of.sum()),
-                "state_flux_checksum_rel":
-                    abs(mp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
-            if f32_run is not None:
-                ff, fp = f32_run
-                acc["mixed_f32_storage_vs_fp64"] = {
-                    "fine_flux_checksum_rel": abs(ff.sum() - of.sum()) / abs(of.sum()),
-                    "state_flux_checksum_rel":
-                        abs(fp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
-            sys.path.insert(0, os.path.join(ROOT, "tests"))
-            from conftest import quantized
-            of, op = pyoracle.sweep_f64(quantized(mprob, storage), pyoracle.EXP_NATIVE,
-                                        nthreads=threads)
-            acc["mixed_vs_fp64_on_dequantised_inputs"] = {
-                "fine_flux_checksum_rel": abs(mf.sum() - of.sum()) / abs(of.sum()),
-                "state_flux_checksum_rel":
-                    abs(mp.astype(np.float64).sum() - op.sum()) / abs(op.sum())}
+            for name, (f, p, storage) in mixed["sorted"].items():
+                acc[f"{name}_vs_fp64_on_fp32_masters"] = errs(f, p, of, op)
+            for name, (f, p, storage) in mixed["sorted"].items():
+                if storage != "f32":
+                    qf, qp = pyoracle.sweep_f64(quantized(mprob, storage), pyoracle.EXP_NATIVE,
+                                                nthreads=threads)
+                    acc[f"{name}_vs_fp64_on_dequantised_inputs"] = errs(f, p, qf, qp)
         acc["oracle_s"] = time.time() - t0
     return base, acc
 
@@ -307,51 +322,86 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        mstore, mtally = "bf16", "det-f64"
-        mprob = smoc.generate(S, G=G, R=R, F=F, seed=42, track_len=L, sorted_stream=True)
-        with smoc.Context(G, R, F, storage=mstore, exp=args.exp, tally=mtally) as ctx:
-            ctx.load(mprob)
-            mwall, msweep, mtotal, mused = timed_steps(ctx, args.steps, max(args.warmup, 1),
-                                                       {"autotune": 1, "use_graph": 1})
-            mf, mp = ctx.results()
-        mvalue = sg / (mwall / args.steps)
-        mB = alg_bytes_per_sg(mstore, mtally, G, L)
-        out["mixed"] = {
-            "workload": "same shape; stream pre-sorted by (region, fine interval), tracks of "
-                        f"{L} consecutive sorted segments; {mstore} storage, FP32 math, runs "
-                        "summed in registers, deterministic FP64 segmented reduction; "
-                        "autotuned launch replayed from a hipGraph",
-            "value": mvalue, "ms_per_step": mwall / args.steps * 1e3,
-            "vs_plain": mvalue / value, "sweep_ms": msweep, "total_ms": mtotal,
-            "hbm_gbs_alg": mB * sg / (mtotal * 1e-3) / 1e9, "alg_bytes_per_sg": mB,
-            "launch": mused,
-            "checksums": dict(zip(("fine_flux", "state_flux"), smoc.checksums(mf, mp)))}
+        tuned = {"autotune": 1, "use_graph": 1}
+        mwarm = max(args.warmup, 1)
+
+        def line(problem, storage, tally, launch, workload):
+            """One timed configuration on its own context: the JSON entry and
+            the results (fine_flux, psi_out)."""
+            with smoc.Context(G, R, F, storage=storage, exp=args.exp, tally=tally) as ctx:
+                ctx.load(problem)
+                lwall, lsweep, ltotal, lused = timed_steps(ctx, args.steps, mwarm, launch)
+                lf, lp = ctx.results()
+            v = sg / (lwall / args.steps)
+            entry = {"workload": workload, "value": v, "ms_per_step": lwall / args.steps * 1e3,
+                     "vs_plain": v / value, "sweep_ms": lsweep, "total_ms": ltotal,
+                     "alg_bytes_per_sg": alg_bytes_per_sg(storage, tally, G, L),
+                     "storage": storage, "tally": tally, "launch": lused,
+                     "checksums": dict(zip(("fine_flux", "state_flux"),
+                                           smoc.checksums(lf, lp)))}
+            return entry, lf, lp
+
+        mprob, sort_s = smoc.sort_stream(prob)
+        out["sort"] = {"host_s": sort_s, "what": f"smoc_sort_segments of the {S}-segment "
+                       "stream by (region, fine interval) on one host thread (stable counting "
+                       "sort); tracks stay the same index ranges, so a sorted track is "
+                       f"{L} consecutive sorted segments; outside every timed region"}
+        sorted_runs = {}
+        out["mixed"], mf, mp = line(
+            mprob, "f32", "det-f64", tuned,
+            f"same shape; stream pre-sorted by (region, fine interval), tracks of {L} "
+            "consecutive sorted segments; FP32 storage and math, runs summed in registers, "
+            "deterministic FP64 segmented reduction; autotuned launch replayed from a hipGraph")
+        sorted_runs["mixed"] = (mf, mp, "f32")
+        mused = out["mixed"]["launch"]
+        mwall_step = out["mixed"]["ms_per_step"] * 1e-3
         # (the profile's launch has 2 groups per lane; the count holds for that shape)
         valu = mixed_valu_instructions() if ((S, G, L) == (50_000_000, 100, 100) and
                                              mused.get("groups_per_lane") == 2) else None
         if valu is not None:
             # lane-operations per second over the whole step (reset + sweep + reduce): a
-            # lower bound for k_sweep's own rate
-            ach = valu * 64 / (mwall / args.steps)
+            # lower bound for k_sweep's own rate; useful: the G of the 128 group slots a
+            # wave carries at 2 groups per lane
+            ach = valu * 64 / mwall_step
+            slots = -(-G // 128) * 128
             out["mixed"]["roofline"] = {
                 "bound": "valu", "achieved": ach, "peak": VALU_PEAK_LANE_OPS,
                 "unit": "VALU lane-ops/s", "frac": ach / VALU_PEAK_LANE_OPS,
+                "useful_frac": ach * G / slots / VALU_PEAK_LANE_OPS,
                 "kernel": "k_sweep", "valu_instructions_per_launch": valu,
                 "source": os.path.relpath(MIXED_SUMMARY, ROOT)}
-        # the same sorted path with FP32 storage: no quantisation, so its error
-        # against FP64 on the FP32 masters is the arithmetic's alone
-        with smoc.Context(G, R, F, storage="f32", exp=args.exp, tally=mtally) as ctx:
-            ctx.load(mprob)
-            fwall, fsweep, ftotal, fused = timed_steps(ctx, args.steps, max(args.warmup, 1),
-                                                       {"autotune": 1, "use_graph": 1})
-            ff, fp = ctx.results()
-        fvalue = sg / (fwall / args.steps)
-        out["mixed_f32_storage"] = {
-            "workload": "as mixed, with FP32 storage of sigT / fine_source",
-            "value": fvalue, "ms_per_step": fwall / args.steps * 1e3,
-            "vs_plain": fvalue / value, "sweep_ms": fsweep, "total_ms": ftotal,
-            "launch": fused}
-        mixed = (mprob, mf, mp, mstore, (ff, fp))
+        for st in ("bf16", "f16"):
+            key = f"mixed_{st}_storage"
+            out[key], f16f, f16p = line(mprob, st, "det-f64", tuned,
+                                        f"as mixed, with {st} storage of sigT / fine_source")
+            sorted_runs[key] = (f16f, f16p, st)
+        out["plain_sorted"], f_, p_ = line(
+            mprob, "f32", "atomic-f32", {},
+            "the plain configuration (FP32 storage, atomic FP32 tally, default launch) on the "
+            "sorted stream: runs summed in registers, one atomic flush per run")
+        sorted_runs["plain_sorted"] = (f_, p_, "f32")
+        out["plain_sorted_segment_flush"], f_, p_ = line(
+            mprob, "f32", "atomic-f32", {"segment_flush": 1},
+            "the plain kernel itself on the sorted stream: one atomic flush per segment "
+            "(segment_flush), as on the track-order stream")
+        sorted_runs["plain_sorted_segment_flush"] = (f_, p_, "f32")
+        out["mixed_track_order"], tf, tp = line(
+            prob, "f32", "det-f64", tuned,
+            "the mixed configuration (FP32 storage, deterministic FP64 tally, autotuned graph) "
+            "on the plain problem's own track-order stream: every run is one segment, so "
+            "per-segment FP32 partials and the fixed-order segmented reduction")
+        v = {k: out[k]["value"] for k in ("plain_sorted_segment_flush", "plain_sorted", "mixed",
+                                          "mixed_track_order")}
+        out["attribution"] = {
+            "stream_order_alone": v["plain_sorted_segment_flush"] / value,
+            "run_sums_on_sorted_stream": v["plain_sorted"] / v["plain_sorted_segment_flush"],
+            "det_tally_graph_autotune": v["mixed"] / v["plain_sorted"],
+            "mixed_over_plain": v["mixed"] / value,
+            "mixed_without_sorting": v["mixed_track_order"] / value,
+            "note": "ratios of attenuations/s, each changing one thing: plain -> plain kernel "
+                    "on the sorted stream -> run sums -> deterministic FP64 tally + autotuned "
+                    "graph (= mixed); and the mixed configuration on the unsorted stream"}
+        mixed = {"prob": mprob, "sorted": sorted_runs, "track_order": (tf, tp)}
 
     if not args.no_cpu_baseline:
         base, acc = cpu_leg(prob, args.cpu_sample_segments, args.cpu_single_segments,

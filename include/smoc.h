@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SMOC_ABI_VERSION 1
+#define SMOC_ABI_VERSION 2
 
 /* ---- error codes ------------------------------------------------------ */
 #define SMOC_OK 0
@@ -105,8 +105,9 @@ typedef struct {
 /* Launch configuration.  Zero fields mean "choose": the library's default
  * or, with autotune = 1, the fastest candidate measured on this problem.
  * None of these fields changes results except chunk (splitting a track into
- * chunks re-associates its psi recurrence through the affine scan, a
- * rounding-level change). */
+ * chunks re-associates its psi recurrence through the affine scan) and
+ * segment_flush (tallies summed per segment instead of per run), both
+ * rounding-level changes. */
 typedef struct {
     int seg_per_block;   /* -p: segments per work item; used as chunk when
                             chunk is 0 (0 = whole tracks)                     */
@@ -120,6 +121,14 @@ typedef struct {
     int reduce_waves;    /* deterministic tallies: wavefronts per workgroup of
                             the segmented reduction (1..16; default 4); the
                             summation order does not depend on it           */
+    int segment_flush;   /* 1: tally every segment on its own even when the
+                            stream is region-sorted (the loop a stream in
+                            generation order takes), instead of summing runs
+                            in registers; a measurement switch that shows
+                            what the run sums save.  Atomic tallies and
+                            FIXED64 only: on a region-sorted stream the
+                            deterministic tallies refuse it (SMOC_ERR_ARG),
+                            since their run bookkeeping follows the stream */
 } smoc_launch;
 
 typedef struct {

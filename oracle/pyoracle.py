@@ -44,34 +44,40 @@ def lib():
     if _lib is None:
         if not os.path.exists(ORACLE_LIB):
             raise FileNotFoundError(f"{ORACLE_LIB} not built; run `make -C {HERE}`")
-        L = ctypes.CDLL(ORACLE_LIB)
-        TP = ctypes.POINTER(Table)
-        L.oracle_build_table.argtypes = [ctypes.c_float, ctypes.c_float, TP]
-        L.oracle_build_table.restype = ctypes.c_int
-        L.oracle_table_lookup_f32.argtypes = [TP, ctypes.c_float]
-        L.oracle_table_lookup_f32.restype = ctypes.c_float
-        L.oracle_table_interval.argtypes = [TP, ctypes.c_float]
-        L.oracle_table_interval.restype = ctypes.c_int
-        L.oracle_decode_z.argtypes = [ctypes.c_float, ctypes.c_int,
-                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]
-        L.oracle_decode_z.restype = None
-        L.oracle_segment_f32.argtypes = [ctypes.c_int, ctypes.c_int, _F32, _F32, ctypes.c_int,
-                                         ctypes.c_float, ctypes.c_float, ctypes.c_int, TP,
-                                         _F32, _F32]
-        L.oracle_segment_f32.restype = None
-        L.oracle_segment_f64.argtypes = [ctypes.c_int, ctypes.c_int, _F32, _F32, ctypes.c_int,
-                                         ctypes.c_float, ctypes.c_float, ctypes.c_int, TP,
-                                         _F64, _F64]
-        L.oracle_segment_f64.restype = None
-        PP = ctypes.POINTER(Problem)
-        L.oracle_sweep_f64.argtypes = [PP, _F64, _F64, ctypes.c_int]
-        L.oracle_sweep_f64.restype = None
-        L.oracle_sweep_f32.argtypes = [PP, _F32, _F32]
-        L.oracle_sweep_f32.restype = None
-        L.oracle_baseline_f32.argtypes = [PP, _F32, _F32, ctypes.c_int]
-        L.oracle_baseline_f32.restype = ctypes.c_double
-        _lib = L
+        _lib = load(ORACLE_LIB)
     return _lib
+
+
+def load(path):
+    """A build of smoc_oracle.c at `path` with its signatures bound (the
+    checker itself, or a mutant compiled by a test)."""
+    L = ctypes.CDLL(path)
+    TP = ctypes.POINTER(Table)
+    L.oracle_build_table.argtypes = [ctypes.c_float, ctypes.c_float, TP]
+    L.oracle_build_table.restype = ctypes.c_int
+    L.oracle_table_lookup_f32.argtypes = [TP, ctypes.c_float]
+    L.oracle_table_lookup_f32.restype = ctypes.c_float
+    L.oracle_table_interval.argtypes = [TP, ctypes.c_float]
+    L.oracle_table_interval.restype = ctypes.c_int
+    L.oracle_decode_z.argtypes = [ctypes.c_float, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]
+    L.oracle_decode_z.restype = None
+    L.oracle_segment_f32.argtypes = [ctypes.c_int, ctypes.c_int, _F32, _F32, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_int, TP,
+                                     _F32, _F32]
+    L.oracle_segment_f32.restype = None
+    L.oracle_segment_f64.argtypes = [ctypes.c_int, ctypes.c_int, _F32, _F32, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_int, TP,
+                                     _F64, _F64]
+    L.oracle_segment_f64.restype = None
+    PP = ctypes.POINTER(Problem)
+    L.oracle_sweep_f64.argtypes = [PP, _F64, _F64, ctypes.c_int]
+    L.oracle_sweep_f64.restype = None
+    L.oracle_sweep_f32.argtypes = [PP, _F32, _F32]
+    L.oracle_sweep_f32.restype = None
+    L.oracle_baseline_f32.argtypes = [PP, _F32, _F32, ctypes.c_int]
+    L.oracle_baseline_f32.restype = ctypes.c_double
+    return L
 
 
 def table(precision=0.01, max_val=10.0) -> Table:

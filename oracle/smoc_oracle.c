@@ -146,6 +146,14 @@ void oracle_segment_f32(int G, int F, const float *sig, const float *src, int fa
     }
 }
 
+/* The FP64 flux integral's q2 term (kernel.c:250-252) is multiplied by this
+ * constant, 1 in every build of the checker.  tests/test_oracle.py compiles a
+ * mutant with -DORACLE_Q2_TERM_SCALE=2 to show that its pin of this term
+ * against the reference's golden tallies fails on such a slip. */
+#ifndef ORACLE_Q2_TERM_SCALE
+#define ORACLE_Q2_TERM_SCALE 1.0
+#endif
+
 void oracle_segment_f64(int G, int F, const float *sig, const float *src, int fai,
                         float ds_f, float zin_f, int exp_mode, const oracle_table *tab,
                         double *psi, double *flux)
@@ -197,7 +205,7 @@ void oracle_segment_f64(int G, int F, const float *sig, const float *src, int fa
         const double reuse = tau * (tau - 2.0) + 2.0 * expVal / (sigT * sigT2);
         const double flux_integral =
             (q0 * tau + (sigT * psi[g] - q0) * expVal) / sigT2 + q1 * mu * reuse +
-            q2 * mu2 * (tau * (tau * (tau - 3.0) + 6.0) - 6.0 * expVal) /
+            ORACLE_Q2_TERM_SCALE * q2 * mu2 * (tau * (tau * (tau - 3.0) + 6.0) - 6.0 * expVal) /
                 (3.0 * sigT2 * sigT2);
         FSR_flux[g] += weight * flux_integral;
         const double t1 = q0 * expVal / sigT;

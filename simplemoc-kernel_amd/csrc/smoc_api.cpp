@@ -160,6 +160,16 @@ struct smoc_ctx {
     bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
 };
 
+// A captured graph holds the device addresses of every buffer its kernels
+// touch; whatever frees or reallocates one of them drops the graph first.
+static void drop_graph(smoc_ctx *c)
+{
+    if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+}
+
 // Stream and events of a measurement probe, released on every return path.
 struct ProbeTimer {
     hipStream_t s = nullptr;
@@ -501,6 +511,14 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
             return SMOC_ERR_ARG;
         }
     const int64_t nrow = c->n_fsr();
+    // the buffers below are reallocated: until this call succeeds the context
+    // holds no region data, and no captured graph may replay on freed memory
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    c->regions_set = false;
+    c->ran = false;
+    c->tuned_valid = false;
+    c->scatter_set = false;
     int rc = c->tuples.alloc((size_t)nrow * c->Gp * smoc_tuple_bytes(c->p.storage));
     if (rc == SMOC_OK)
         rc = c->row_thr.alloc((size_t)nrow);
@@ -534,9 +552,6 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
         c->flux_init.release();
     }
     c->regions_set = true;
-    c->ran = false;
-    c->tuned_valid = false;
-    c->scatter_set = false;
     return SMOC_OK;
 }
 
@@ -665,6 +680,18 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
         c->fix_limit = (float)(2147483647.0 / most * 0.999);
     }
 
+    // the buffers below are reallocated: until this call succeeds the context
+    // holds no segment data, no chunk plan and no captured graph
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    c->segments_set = false;
+    c->ran = false;
+    c->tuned_valid = false;
+    c->plan.release();
+    c->S = c->T = 0;
+    c->n_runs = 0;
+    c->max_track = 0;
+    c->h_track_off.clear();
     // the stream arrays carry kStreamPad zeroed entries past S: the sweep
     // loads one segment ahead without a bounds branch
     int rc = c->len.alloc((size_t)S + kStreamPad);
@@ -712,26 +739,24 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
             return rc;
         HIPCHK(hipMemcpy(c->vol.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
     }
-    c->S = S;
-    c->T = T;
-    c->n_runs = 0;
-    c->h_track_off.assign(track_off, track_off + T + 1);
-    c->max_track = 0;
-    for (int64_t t = 0; t < T; t++)
-        c->max_track = std::max<int64_t>(c->max_track, track_off[t + 1] - track_off[t]);
-    c->plan.release();
-    if (c->graph_exec) {
-        (void)hipGraphExecDestroy(c->graph_exec);
-        c->graph_exec = nullptr;
-    }
-    c->tuned_valid = false;
     if (c->det()) {
         rc = build_runs(c, S, region, z, T, track_off);
         if (rc != SMOC_OK)
             return rc;
     }
+    try {
+        c->h_track_off.assign(track_off, track_off + T + 1);
+    } catch (...) {
+        smoc_set_error("smoc_set_segments: host allocation for track offsets failed");
+        return SMOC_ERR_NOMEM;
+    }
+    int64_t longest = 0;
+    for (int64_t t = 0; t < T; t++)
+        longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
+    c->max_track = longest;
+    c->S = S;
+    c->T = T;
     c->segments_set = true;
-    c->ran = false;
     return SMOC_OK;
 }
 
@@ -756,6 +781,7 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
     d.use_graph = d.use_graph ? 1 : 0;
     if (d.reduce_waves < 1 || d.reduce_waves > 16)
         d.reduce_waves = 4;
+    d.segment_flush = d.segment_flush ? 1 : 0;
     *l = d;
 }
 
@@ -767,6 +793,13 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
 static int build_plan(smoc_ctx *c, int C)
 {
     Plan &pl = c->plan;
+    // the plan buffers are reallocated below; a graph captured with the old
+    // ones (any chunk length) must not replay, and until this plan is complete
+    // no chunk length is planned
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    pl.chunk = 0;
+    pl.n_items = pl.n1 = pl.n2 = 0;
     std::vector<int64_t> items, tio((size_t)c->T + 1);
     std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
     const std::vector<int64_t> &off = c->h_track_off;
@@ -841,9 +874,16 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     const bool chunked = l.chunk > 0;
     // FIXED64 too: a chunk boundary would split a run's sum, and the run sums
     // are what is rounded to fixed point, so results would depend on the chunk
-    if (chunked && (c->det() || c->fixed()) && c->runs) {
+    // per-run sums unless the caller asks for a tally per segment
+    const bool runs = c->runs && !l.segment_flush;
+    if (chunked && (c->det() || c->fixed()) && runs) {
         smoc_set_error("smoc_run: chunked sweeps of a region-sorted stream need an atomic "
                        "FP32/FP64 tally (deterministic runs would cross chunk boundaries)");
+        return SMOC_ERR_ARG;
+    }
+    if (l.segment_flush && c->det() && c->runs) {
+        smoc_set_error("smoc_run: segment_flush on a region-sorted stream needs an atomic or "
+                       "FIXED64 tally (the deterministic run bookkeeping follows the stream)");
         return SMOC_ERR_ARG;
     }
     if (chunked && c->plan.chunk != l.chunk) {
@@ -876,7 +916,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     a.Gp = c->Gp;
     a.F = c->p.fine_axial;
     a.slices = slices;
-    a.runs = c->runs ? 1 : 0;
+    a.runs = runs ? 1 : 0;
     a.fix_limit = c->fix_limit;
     // persistent grids: as many workgroups as can be resident (8 waves per
     // SIMD, 4 SIMDs per CU)
@@ -973,7 +1013,7 @@ static bool same_launch(const smoc_launch &x, const smoc_launch &y)
 {
     return x.groups_per_lane == y.groups_per_lane && x.waves_per_block == y.waves_per_block &&
            x.chunk == y.chunk && x.persistent == y.persistent &&
-           x.reduce_waves == y.reduce_waves;
+           x.reduce_waves == y.reduce_waves && x.segment_flush == y.segment_flush;
 }
 
 // One run, timed with events on the context stream; with use_graph the

@@ -203,8 +203,7 @@ int main(int argc, char *argv[])
 {
     options o;
     memset(&o, 0, sizeof(o));
-    smoc_default_input(&o.in);
-    o.in.egroups = 100; /* the north-star default shape; -e overrides */
+    smoc_default_input(&o.in); /* 50,000,000 segments x 128 groups, as the reference */
     o.track_len = 100;
     o.seed = 42;
     o.steps = 1;

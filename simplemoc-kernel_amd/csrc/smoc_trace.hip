@@ -298,6 +298,14 @@ extern "C" int smoc_trace_segments(int device, const smoc_geometry *g, uint64_t 
         smoc_set_error("smoc_trace_segments: track_off must come from smoc_trace_count");
         return SMOC_ERR_ARG;
     }
+    // k_trace_fill writes track t's segments at [track_off[t], track_off[t+1]),
+    // bounded by track_off[t+1] only: offsets that decrease would let a track
+    // write past the S-sized buffers, so they are refused here
+    for (int64_t t = 0; t < T; t++)
+        if (track_off[t + 1] < track_off[t]) {
+            smoc_set_error("smoc_trace_segments: track_off not monotone at %lld", (long long)t);
+            return SMOC_ERR_ARG;
+        }
     const int64_t S = track_off[T];
     if (S > 0 && (!len || !region || !z)) {
         smoc_set_error("smoc_trace_segments: NULL output");

@@ -48,7 +48,8 @@ class Launch(ctypes.Structure):
     _fields_ = [("seg_per_block", ctypes.c_int), ("groups_per_lane", ctypes.c_int),
                 ("waves_per_block", ctypes.c_int), ("chunk", ctypes.c_int),
                 ("persistent", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("autotune", ctypes.c_int), ("reduce_waves", ctypes.c_int)]
+                ("autotune", ctypes.c_int), ("reduce_waves", ctypes.c_int),
+                ("segment_flush", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -281,6 +282,19 @@ def generate(S: int, G: int = 100, R: int = 6750, F: int = 5, seed: int = 42,
     psi = np.empty((T, G), np.float32)
     _check(L.smoc_generate_psi(seed, T, G, psi), "smoc_generate_psi")
     return Problem(G, R, F, sig, src, ln, rg, zz, off, psi, seed)
+
+
+def sort_stream(prob: Problem):
+    """(copy of prob with the segment stream sorted by (region, fine interval),
+    host seconds of smoc_sort_segments).  The sort is stable and the tracks
+    stay the same index ranges of the stream, as generate(sorted_stream=True)."""
+    import copy
+    import time
+    q = copy.copy(prob)
+    q.len, q.region, q.z = prob.len.copy(), prob.region.copy(), prob.z.copy()
+    t0 = time.perf_counter()
+    _check(lib().smoc_sort_segments(q.S, q.R, q.F, q.len, q.region, q.z), "smoc_sort_segments")
+    return q, time.perf_counter() - t0
 
 
 class Context:

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_errors():
     L = smoc.lib()
-    assert L.smoc_abi_version() == 1
+    assert L.smoc_abi_version() == 2
     for code, word in ((0, b"ok"), (-1, b"invalid"), (-2, b"device"), (-3, b"HIP"),
                        (-4, b"order"), (-5, b"memory"), (-99, b"unknown")):
         assert word in L.smoc_strerror(code)

@@ -38,6 +38,22 @@ def test_no_device_is_an_error():
     assert "no usable HIP device" in r.stderr
 
 
+def test_defaults_are_the_reference_problem():
+    """With no options the driver sets up the reference's default problem
+    (src/cpu/init.c:4-24, main.c:18-19): 50,000,000 segments x 128 groups over
+    6750 derived source regions.  The summary is printed before the device is
+    opened, so it is checked with or without a GPU; with one the run completes."""
+    r = run(timeout=600)
+    for line in ("Energy Groups:           128", "3D Source Regions:       6750",
+                 "Segments:                50,000,000", "Fine Axial Intervals:    5"):
+        assert line in r.stdout, (line, r.stdout[-1500:])
+    if smoc.device_count() > 0:
+        assert r.returncode == 0, r.stderr
+        assert "Time per Intersection:" in r.stdout
+    else:
+        assert r.returncode == 2 and "no usable HIP device" in r.stderr
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)

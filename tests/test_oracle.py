@@ -92,6 +92,61 @@ def test_segment_f64_vs_f32_on_golden():
     assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < 1e-5
 
 
+def _tally_f64(L, d):
+    """The FP64 oracle's fine_flux increments over the fixture's 64 segments
+    (reference constants ds = 0.7, zin = 0.3: tau = sigT ds reaches 7, where
+    the quadratic-source term of the flux integral is not negligible)."""
+    tab = pyoracle.table()
+    G, F = int(d["seg_G"]), int(d["seg_F"])
+    flux0 = d["seg_fine_flux_in"].astype(np.float64)
+    flux = flux0.copy()
+    psi = d["seg_psi_in"].astype(np.float64)
+    for r, f in zip(d["seg_region"], d["seg_fai"]):
+        blk = np.ascontiguousarray(flux[r])
+        L.oracle_segment_f64(G, F, np.ascontiguousarray(d["seg_sigT"][r]),
+                             np.ascontiguousarray(d["seg_fine_source"][r]), int(f), 0.7, 0.3, 0,
+                             tab, psi, blk)
+        flux[r] = blk
+    return flux - flux0
+
+
+def _golden_tally(d):
+    return d["flux_exp_strict"].astype(np.float64) - d["seg_fine_flux_in"].astype(np.float64)
+
+
+# normwise bound of the FP64 tally against the reference's strict FP32 build:
+# measured 5.8e-7 (the FP32 rounding of the reference); the q2 term of the flux
+# integral (kernel.c:250-252) is 4.2e-4 of the tally here, so a dropped,
+# doubled or sign-flipped q2 term is 700x beyond the bound
+TALLY_PIN = 2e-6
+
+
+def test_segment_f64_tally_vs_reference_golden():
+    """The FP64 oracle's fine_flux tally -- every term of the flux integral,
+    kernel.c:245-252, weighted as kernel.c:259-263 -- against the reference's
+    own tallies on the golden vectors."""
+    inc, ref = _tally_f64(pyoracle.lib(), GOLD), _golden_tally(GOLD)
+    assert np.linalg.norm(inc - ref) / np.linalg.norm(ref) < TALLY_PIN
+
+
+def test_tally_pin_catches_a_doubled_q2_term(tmp_path):
+    """A mutant oracle whose q2 term is doubled (compiled here from the same
+    source with -DORACLE_Q2_TERM_SCALE=2) fails the pin above."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler to build the mutant")
+    src = os.path.join(os.path.dirname(pyoracle.__file__), "smoc_oracle.c")
+    out = str(tmp_path / "liboracle_q2x2.so")
+    subprocess.run([cc, "-std=gnu99", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+                    "-fPIC", "-shared", "-DORACLE_Q2_TERM_SCALE=2.0", src, "-o", out, "-lm"],
+                   check=True)
+    inc, ref = _tally_f64(pyoracle.load(out), GOLD), _golden_tally(GOLD)
+    err = np.linalg.norm(inc - ref) / np.linalg.norm(ref)
+    assert err > 100 * TALLY_PIN, err
+
+
 @pytest.mark.skipif(not pyoracle.ref_available("exp_strict"), reason="oracle/_ref not built")
 @pytest.mark.parametrize("mode,variant", [(0, "exp_strict"), (1, "table_strict")])
 def test_segment_f32_bitexact_vs_live_reference(mode, variant):
