@@ -69,6 +69,11 @@ MIXED_SUMMARY = os.path.join(ROOT, "profiles", "k_sweep_mixed_sorted_f32_50Mx100
 VALU_PEAK_LANE_OPS = 157.3e12 / 2
 
 
+def log(msg: str):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def alg_bytes_per_sg(storage: str, tally: str, G: int, L: float) -> float:
     """Reference-formulation bytes per segment-group: sigT + the 3-point source
     stencil + tally read and write + the segment stream (12 B per segment,
@@ -101,6 +106,8 @@ def parse():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (counter "
                          "calibration) and print its bandwidth")
+    ap.add_argument("--copy-variant", type=int, default=0,
+                    help="copy kernel variant for --copy-probe-mib (-1: every variant)")
     return ap.parse_args()
 
 
@@ -243,9 +250,10 @@ def main():
         sys.exit(2)
     if args.copy_probe_mib:
         nbytes = args.copy_probe_mib << 20
-        ms = smoc.probe_copy(nbytes, reps=5)
-        print(json.dumps({"copy_probe_bytes": nbytes, "mean_ms": ms,
-                          "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
+        for var in ([args.copy_variant] if args.copy_variant >= 0 else range(4)):
+            ms = smoc.probe_copy(nbytes, reps=5, variant=var)
+            print(json.dumps({"copy_probe_bytes": nbytes, "variant": var, "mean_ms": ms,
+                              "gbs_read_plus_write": 2 * nbytes / ms / 1e6}), flush=True)
         return
     G, S, L = args.groups, args.segments, args.track_len
     R, F = 6750, 5
@@ -259,6 +267,7 @@ def main():
     if args.wpb:
         launch["waves_per_block"] = args.wpb
 
+    log(f"plain: {S} x {G}, {args.storage} storage, {args.tally}")
     with smoc.Context(G, R, F, storage=args.storage, exp=args.exp, tally=args.tally) as ctx:
         t0 = time.perf_counter()
         ctx.load(prob)  # host buffers -> HBM, stream decode; returns synchronised
@@ -328,6 +337,7 @@ def main():
         def line(problem, storage, tally, launch, workload):
             """One timed configuration on its own context: the JSON entry and
             the results (fine_flux, psi_out)."""
+            log(f"{storage} storage, {tally}, launch {launch}")
             with smoc.Context(G, R, F, storage=storage, exp=args.exp, tally=tally) as ctx:
                 ctx.load(problem)
                 lwall, lsweep, ltotal, lused = timed_steps(ctx, args.steps, mwarm, launch)
@@ -404,6 +414,7 @@ def main():
         mixed = {"prob": mprob, "sorted": sorted_runs, "track_order": (tf, tp)}
 
     if not args.no_cpu_baseline:
+        log("CPU baseline and FP64 oracle")
         base, acc = cpu_leg(prob, args.cpu_sample_segments, args.cpu_single_segments,
                             (flux, psi), mixed,
                             not args.no_accuracy)

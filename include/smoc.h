@@ -260,8 +260,14 @@ int smoc_device_info(int device, char *name, int name_cap, int *compute_units);
 /* Measurement utility (no reference counterpart): copies `bytes` bytes
  * between two device buffers with a plain vectorised copy kernel `reps`
  * times and returns the mean kernel time.  Used to calibrate HBM traffic
- * counters and to state the achievable copy bandwidth next to the roofline. */
-int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms);
+ * counters and to measure the attainable copy bandwidth next to the roofline.
+ * variant (SMOC_COPY_VARIANTS of them) picks the copy kernel:
+ *   0  4 x 16 B per lane, one pass, one 256-thread workgroup per 16 KiB
+ *   1  8 x 16 B per lane in flight, grid-stride over 8 workgroups per CU
+ *   2  as 1 with nontemporal loads and stores
+ *   3  16 x 16 B per lane, one pass, one workgroup per 64 KiB */
+#define SMOC_COPY_VARIANTS 4
+int smoc_probe_copy(int device, int64_t bytes, int reps, int variant, double *mean_ms);
 
 /* Measurement utility (no reference counterpart): the unsorted sweep's
  * per-segment FP32 atomic tally alone -- S segments of 100 per wavefront, each

@@ -243,10 +243,12 @@ int smoc_device_info(int device, char *name, int name_cap, int *compute_units)
     return SMOC_OK;
 }
 
-int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
+int smoc_probe_copy(int device, int64_t bytes, int reps, int variant, double *mean_ms)
 {
-    if (bytes <= 0 || bytes % 16 != 0 || reps < 1 || !mean_ms) {
-        smoc_set_error("smoc_probe_copy: bytes must be a positive multiple of 16, reps >= 1");
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1 || !mean_ms || variant < 0 ||
+        variant >= SMOC_COPY_VARIANTS) {
+        smoc_set_error("smoc_probe_copy: bytes must be a positive multiple of 16, reps >= 1, "
+                       "0 <= variant < %d", SMOC_COPY_VARIANTS);
         return SMOC_ERR_ARG;
     }
     const int64_t nblk = (bytes / 16 + 1023) / 1024;
@@ -271,10 +273,11 @@ int smoc_probe_copy(int device, int64_t bytes, int reps, double *mean_ms)
         return rc;
     HIPCHK(hipMemset(a.p, 1, (size_t)bytes));
     HIPCHK(hipDeviceSynchronize());
-    const int grid = (int)nblk;
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     return time_probe(
-        [&](hipStream_t s) { return smoc_launch_copy(a.p, b.p, bytes / 16, grid, s); }, reps,
-        mean_ms);
+        [&](hipStream_t s) { return smoc_launch_copy(a.p, b.p, bytes / 16, variant, cus, s); },
+        reps, mean_ms);
 }
 
 int smoc_probe_tally(int device, int64_t S, int rows, int G, int reps, double *mean_ms)

@@ -117,7 +117,9 @@ int64_t smoc_update_blocks(int64_t n);
 int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *stream);
 int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
-int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream);
+/* plain copy of n16 16-byte elements, kernel variant 0..SMOC_COPY_VARIANTS-1 */
+int smoc_launch_copy(const void *src, void *dst, int64_t n16, int variant, int cus,
+                     void *stream);
 /* k_tally_probe over S segments into [rows][Gp] FP32; grid = ceil(ceil(S / 100) / 4) */
 int smoc_launch_tally_probe(void *flux, int64_t S, int rows, int G, int Gp, int grid,
                             void *stream);

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "smoc.h"
@@ -1141,20 +1142,33 @@ __global__ void __launch_bounds__(256) k_reset(T *flux, const float *init, int64
         flux[i] = init ? (T)init[i] : (T)0;
 }
 
-// Plain copy, 4 independent 16-byte loads in flight per lane.
-__global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ src,
-                                               float4 *__restrict__ dst, int64_t n)
+// Plain copy: U independent 16-byte loads in flight per lane, then the U
+// stores; STRIDE: the grid strides over the array (a fixed grid sized to the
+// CUs), else one pass with one workgroup per 256 U elements; NT: nontemporal
+// loads and stores (the copy's data is touched once).
+typedef float vec4f __attribute__((ext_vector_type(4)));
+template <int U, bool STRIDE, bool NT>
+__global__ void __launch_bounds__(256) k_copy(const vec4f *__restrict__ src,
+                                               vec4f *__restrict__ dst, int64_t n)
 {
-    const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-    float4 v[4];
+    const int64_t step = (int64_t)gridDim.x * 256 * U;
+    for (int64_t base = (int64_t)blockIdx.x * 256 * U + threadIdx.x; base < n; base += step) {
+        vec4f v[U];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-        if (base + u * 256 < n)
-            v[u] = src[base + u * 256];
+        for (int u = 0; u < U; u++)
+            if (base + u * 256 < n)
+                v[u] = NT ? __builtin_nontemporal_load(src + base + u * 256) : src[base + u * 256];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-        if (base + u * 256 < n)
-            dst[base + u * 256] = v[u];
+        for (int u = 0; u < U; u++)
+            if (base + u * 256 < n) {
+                if (NT)
+                    __builtin_nontemporal_store(v[u], dst + base + u * 256);
+                else
+                    dst[base + u * 256] = v[u];
+            }
+        if (!STRIDE)
+            break;
+    }
 }
 
 // Tally-only probe: the per-segment FP32 atomic flush of the sweep on an
@@ -1391,12 +1405,23 @@ extern "C" int smoc_launch_reset(void *flux, const float *init, int64_t n, int f
     return (int)hipGetLastError();
 }
 
-extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int grid, void *stream)
+extern "C" int smoc_launch_copy(const void *src, void *dst, int64_t n16, int variant, int cus,
+                                void *stream)
 {
     if (n16 == 0)
         return 0;
-    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const vec4f *a = static_cast<const vec4f *>(src);
+    vec4f *b = static_cast<vec4f *>(dst);
+    const unsigned stride_grid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)cus * 8, (n16 + 2047) / 2048));
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_copy<4, false, false>), dim3(blocks_for(n16, 1024)), dim3(256), 0, s, a, b, n16); break;
+    case 1: hipLaunchKernelGGL((k_copy<8, true, false>), dim3(stride_grid), dim3(256), 0, s, a, b, n16); break;
+    case 2: hipLaunchKernelGGL((k_copy<8, true, true>), dim3(stride_grid), dim3(256), 0, s, a, b, n16); break;
+    case 3: hipLaunchKernelGGL((k_copy<16, false, false>), dim3(blocks_for(n16, 4096)), dim3(256), 0, s, a, b, n16); break;
+    default: return (int)hipErrorInvalidValue;
+    }
     return (int)hipGetLastError();
 }
 

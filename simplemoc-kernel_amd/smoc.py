@@ -132,7 +132,7 @@ SIGNATURES = {
                                   ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "smoc_device_count": (_i, [ctypes.POINTER(_i)]),
     "smoc_device_info": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(_i)]),
-    "smoc_probe_copy": (_i, [_i, _i64, _i, ctypes.POINTER(_d)]),
+    "smoc_probe_copy": (_i, [_i, _i64, _i, _i, ctypes.POINTER(_d)]),
     "smoc_probe_tally": (_i, [_i, _i64, _i, _i, _i, ctypes.POINTER(_d)]),
     "smoc_create": (_i, [ctypes.POINTER(_P), ctypes.POINTER(Params)]),
     "smoc_destroy": (_i, [_P]),
@@ -209,10 +209,11 @@ def device_info(device: int = 0):
     return name.value.decode(), cus.value
 
 
-def probe_copy(nbytes: int, reps: int = 5, device: int = 0) -> float:
+def probe_copy(nbytes: int, reps: int = 5, device: int = 0, variant: int = 0) -> float:
     """Mean time (ms) of the plain copy kernel over nbytes (read + write)."""
     ms = _d()
-    _check(lib().smoc_probe_copy(device, nbytes, reps, ctypes.byref(ms)), "smoc_probe_copy")
+    _check(lib().smoc_probe_copy(device, nbytes, reps, variant, ctypes.byref(ms)),
+           "smoc_probe_copy")
     return ms.value
 
 

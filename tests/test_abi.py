@@ -77,7 +77,8 @@ def test_probes_reject_bad_arguments_before_touching_a_device():
                  (0, 400 * (1 << 31) * 100, 10, 100, 1)):  # grid beyond 2^31 - 1 blocks
         assert L.smoc_probe_tally(*args, ctypes.byref(ms)) == smoc.ERR_ARG, args
     assert L.smoc_probe_tally(0, 100, 10, 100, 1, None) == smoc.ERR_ARG
-    for args in ((0, 0, 1), (0, 24, 1), (0, 1 << 20, 0)):
+    for args in ((0, 0, 1, 0), (0, 24, 1, 0), (0, 1 << 20, 0, 0), (0, 1 << 20, 1, -1),
+                 (0, 1 << 20, 1, 4)):
         assert L.smoc_probe_copy(*args, ctypes.byref(ms)) == smoc.ERR_ARG, args
 
 

@@ -10,9 +10,12 @@ import smoc
 pytestmark = pytest.mark.gpu
 
 
-def test_copy_probe_runs(gpu_ready):
-    ms = smoc.probe_copy(64 << 20, reps=3)
-    print(f"  copy 64 MiB: {ms:.4f} ms, {2 * (64 << 20) / ms / 1e6:.0f} GB/s", flush=True)
+@pytest.mark.parametrize("variant", range(4))
+@pytest.mark.parametrize("nbytes", [64 << 20, (64 << 20) + 48])
+def test_copy_probe_runs(gpu_ready, variant, nbytes):
+    ms = smoc.probe_copy(nbytes, reps=3, variant=variant)
+    print(f"  copy variant {variant}, {nbytes} B: {ms:.4f} ms, {2 * nbytes / ms / 1e6:.0f} GB/s",
+          flush=True)
     assert math.isfinite(ms) and ms > 0
 
 
