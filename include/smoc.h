@@ -300,6 +300,26 @@ int smoc_set_segments(smoc_ctx *ctx, int64_t S, const float *len, const int32_t 
  * the same state.  timing may be NULL.  launch may be NULL (defaults). */
 int smoc_run(smoc_ctx *ctx, const smoc_launch *launch, smoc_timing *timing);
 
+/* ---- allocation-free sweep ------------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same sweep as smoc_run, but the
+ * scratch that smoc_run allocates in the context on first use -- the
+ * persistent work counters and the chunk plan of a chunked sweep (work items
+ * and lists, each chunk's incoming psi and affine transfer) -- comes from a
+ * fixed device array of SMOC_NOALLOC_SCRATCH_BYTES per GPU, which the library
+ * holds from its first smoc_create on that GPU until the process ends.  The
+ * plan is rebuilt in that array by a device kernel on every call, so the call
+ * allocates neither device nor host memory.  Inputs and outputs are the
+ * context's, as for smoc_run (the smoc_set_* calls allocate, this one does
+ * not).  launch->autotune and launch->use_graph must be 0; a chunk plan that
+ * does not fit the array is refused with SMOC_ERR_ARG, naming the bytes it
+ * needs.  All calls on one GPU share the array and leave their contents in
+ * it: calls that overlap on one GPU (several host threads or contexts at
+ * once) race; calls made one after another do not. */
+#define SMOC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int smoc_run_noalloc(smoc_ctx *ctx, const smoc_launch *launch, smoc_timing *timing);
+
 /* Copy results to the host: fine_flux [R][F][G] as double, psi_out [T][G].
  * Either pointer may be NULL. */
 int smoc_get_results(smoc_ctx *ctx, double *fine_flux, float *psi_out);

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -159,6 +160,48 @@ struct smoc_ctx {
     int64_t *fixed_flux() const { return reinterpret_cast<int64_t *>(flux.p + 256); }
     bool det() const { return p.tally == SMOC_TALLY_DET_F64 || p.tally == SMOC_TALLY_DET_KAHAN; }
 };
+
+// ---- scratch of the allocation-free entry point ---------------------------------
+// One array per GPU in the library's code object: the runtime allocates it
+// when the module loads on a device and it lives until the process ends.  Host
+// code must not use the array's name as an address (that is a host-side
+// stand-in); every use goes through the per-device address that
+// hipGetSymbolAddress returns, looked up once per device by smoc_create.
+static __device__ __align__(256) unsigned char scratch[SMOC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// With `device` current: its scratch address, looked up on first use.
+static int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        smoc_set_error("device %d beyond the %d devices the scratch table holds", device,
+                       kMaxDevices);
+        return SMOC_ERR_NODEV;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device])
+        return SMOC_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        smoc_set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                       hipGetErrorString(e));
+        return SMOC_ERR_HIP;
+    }
+    g_scratch[device] = static_cast<unsigned char *>(ptr);
+    return SMOC_OK;
+}
+
+static unsigned char *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
 
 // A captured graph holds the device addresses of every buffer its kernels
 // touch; whatever frees or reallocates one of them drops the graph first.
@@ -345,6 +388,9 @@ int smoc_create(smoc_ctx **out, const smoc_params *p)
         return SMOC_ERR_NODEV;
     }
     HIPCHK(hipSetDevice(p->device));
+    rc = scratch_init(p->device);
+    if (rc != SMOC_OK)
+        return rc;
     smoc_ctx *c = new (std::nothrow) smoc_ctx();
     if (!c) {
         smoc_set_error("smoc_create: host allocation failed");
@@ -866,30 +912,45 @@ static int build_plan(smoc_ctx *c, int C)
     return SMOC_OK;
 }
 
-// Enqueue one whole sweep on the context stream.  ev (4 events) may be NULL
-// (graph capture): reset | transfer + scan | sweep | reduce.
-static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
+// Device buffers a run uses beyond the context's inputs and outputs: the
+// persistent work counters and, for a chunked sweep, the chunk plan.
+// smoc_run takes them from context allocations (ctx_bufs); smoc_run_noalloc
+// from its scratch array, where the plan is rebuilt on the device at the start
+// of every run (plan_chunk > 0).
+struct RunBufs {
+    int64_t *queue = nullptr;  // [2] work counters of the two sweep phases
+    int64_t *items = nullptr, *list1 = nullptr, *list2 = nullptr, *track_item_off = nullptr;
+    float *chunk_psi = nullptr, *xfer = nullptr;
+    int64_t n_items = 0, n1 = 0, n2 = 0;
+    int plan_chunk = 0;
+};
+
+// Launch combinations the loaded problem cannot take.
+static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *what)
 {
-    const int G = c->p.egroups;
-    const int gpl = l.groups_per_lane, wpb = l.waves_per_block;
-    const int slices = (G + 64 * gpl - 1) / (64 * gpl);
-    const int64_t nrow = c->n_fsr();
-    const bool chunked = l.chunk > 0;
+    const bool runs = c->runs && !l.segment_flush;
     // FIXED64 too: a chunk boundary would split a run's sum, and the run sums
     // are what is rounded to fixed point, so results would depend on the chunk
-    // per-run sums unless the caller asks for a tally per segment
-    const bool runs = c->runs && !l.segment_flush;
-    if (chunked && (c->det() || c->fixed()) && runs) {
-        smoc_set_error("smoc_run: chunked sweeps of a region-sorted stream need an atomic "
-                       "FP32/FP64 tally (deterministic runs would cross chunk boundaries)");
+    if (l.chunk > 0 && (c->det() || c->fixed()) && runs) {
+        smoc_set_error("%s: chunked sweeps of a region-sorted stream need an atomic "
+                       "FP32/FP64 tally (deterministic runs would cross chunk boundaries)",
+                       what);
         return SMOC_ERR_ARG;
     }
     if (l.segment_flush && c->det() && c->runs) {
-        smoc_set_error("smoc_run: segment_flush on a region-sorted stream needs an atomic or "
-                       "FIXED64 tally (the deterministic run bookkeeping follows the stream)");
+        smoc_set_error("%s: segment_flush on a region-sorted stream needs an atomic or "
+                       "FIXED64 tally (the deterministic run bookkeeping follows the stream)",
+                       what);
         return SMOC_ERR_ARG;
     }
-    if (chunked && c->plan.chunk != l.chunk) {
+    return SMOC_OK;
+}
+
+// The context's plan for l.chunk (built if it is another chunk length) and
+// work counters (allocated on first use).
+static int ctx_bufs(smoc_ctx *c, const smoc_launch &l, RunBufs *b)
+{
+    if (l.chunk > 0 && c->plan.chunk != l.chunk) {
         int rc = build_plan(c, l.chunk);
         if (rc != SMOC_OK)
             return rc;
@@ -899,6 +960,34 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
         if (rc != SMOC_OK)
             return rc;
     }
+    *b = RunBufs{};
+    b->queue = c->queue.p;
+    if (l.chunk > 0) {
+        Plan &pl = c->plan;
+        b->items = pl.items.p;
+        b->list1 = pl.list1.p;
+        b->list2 = pl.list2.p;
+        b->track_item_off = pl.track_item_off.p;
+        b->chunk_psi = pl.chunk_psi.p;
+        b->xfer = pl.xfer.p;
+        b->n_items = pl.n_items;
+        b->n1 = pl.n1;
+        b->n2 = pl.n2;
+    }
+    return SMOC_OK;
+}
+
+// Enqueue one whole sweep on the context stream.  ev (4 events) may be NULL
+// (graph capture): reset | plan + transfer + scan | sweep | reduce.
+static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipEvent_t *ev)
+{
+    const int G = c->p.egroups;
+    const int gpl = l.groups_per_lane, wpb = l.waves_per_block;
+    const int slices = (G + 64 * gpl - 1) / (64 * gpl);
+    const int64_t nrow = c->n_fsr();
+    const bool chunked = l.chunk > 0;
+    // per-run sums unless the caller asks for a tally per segment
+    const bool runs = c->runs && !l.segment_flush;
 
     smoc_sweep_args a{};
     a.tuples = c->tuples.p;
@@ -948,33 +1037,35 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     if (ev)
         HIPCHK(hipEventRecord(ev[1], c->stream));
     if (chunked) {
+        if (b.plan_chunk > 0)
+            LAUNCHCHK(smoc_launch_plan(c->track_off.p, c->T, b.plan_chunk, b.items, b.list1,
+                                       b.list2, b.track_item_off, c->stream));
         // phase 1: first chunks (swept from psi_in) + transfers of middle chunks
-        Plan &pl = c->plan;
-        a.items = pl.items.p;
-        a.xfer = pl.xfer.p;
+        a.items = b.items;
+        a.xfer = b.xfer;
         smoc_sweep_args p1 = a;
         p1.xfer_mode = 1;
-        p1.item_list = pl.list1.p;
+        p1.item_list = b.list1;
         p1.chunk_psi = nullptr;
-        p1.n_items = pl.n1;
+        p1.n_items = b.n1;
         if (l.persistent) {
-            p1.queue = c->queue.p;
-            p1.queue_batch = batch_for(pl.n1);
-            LAUNCHCHK(smoc_launch_zero_i64(c->queue.p, 1, c->stream));
+            p1.queue = b.queue;
+            p1.queue_batch = batch_for(b.n1);
+            LAUNCHCHK(smoc_launch_zero_i64(b.queue, 1, c->stream));
         }
         LAUNCHCHK(smoc_launch_sweep(&p1, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
-                                    (int)grid_for(pl.n1), c->stream));
-        if (pl.n2 > 0) {
-            smoc_scan_args sa{pl.track_item_off.p, pl.xfer.p, c->psi_in.p, pl.chunk_psi.p, c->T,
+                                    (int)grid_for(b.n1), c->stream));
+        if (b.n2 > 0) {
+            smoc_scan_args sa{b.track_item_off, b.xfer, c->psi_in.p, b.chunk_psi, c->T,
                               c->Gp, slices};
             LAUNCHCHK(smoc_launch_scan(&sa, gpl, c->stream));
         }
         if (ev)
             HIPCHK(hipEventRecord(ev[2], c->stream));
         // phase 2: the remaining chunks from the scanned psi
-        a.item_list = pl.list2.p;
-        a.chunk_psi = pl.chunk_psi.p;
-        a.n_items = pl.n2;
+        a.item_list = b.list2;
+        a.chunk_psi = b.chunk_psi;
+        a.n_items = b.n2;
     } else {
         a.n_items = c->T;
         if (ev)
@@ -982,7 +1073,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, hipEvent_t *ev)
     }
     if (a.n_items > 0) {
         if (l.persistent) {
-            a.queue = c->queue.p + 1;
+            a.queue = b.queue + 1;
             a.queue_batch = batch_for(a.n_items);
             LAUNCHCHK(smoc_launch_zero_i64(a.queue, 1, c->stream));
         }
@@ -1019,30 +1110,43 @@ static bool same_launch(const smoc_launch &x, const smoc_launch &y)
            x.reduce_waves == y.reduce_waves && x.segment_flush == y.segment_flush;
 }
 
+// One eager run on the given buffers, timed with the context's events.
+static int run_eager(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, smoc_timing *tm)
+{
+    int rc = enqueue_run(c, l, b, c->ev);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipEventSynchronize(c->ev[4]));
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; i++)
+        HIPCHK(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
+    *tm = smoc_timing{};
+    tm->reset_ms = ms[0];
+    tm->xfer_ms = ms[1];
+    tm->sweep_ms = ms[2];
+    tm->reduce_ms = ms[3];
+    tm->total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
+    return SMOC_OK;
+}
+
 // One run, timed with events on the context stream; with use_graph the
 // sequence is captured once into a hipGraph and replayed.
 static int run_once(smoc_ctx *c, const smoc_launch &l, smoc_timing *tm)
 {
+    int rc = check_launch(c, l, "smoc_run");
+    if (rc != SMOC_OK)
+        return rc;
+    RunBufs b;
     if (l.use_graph) {
         if (!c->graph_exec || !same_launch(c->graph_launch, l)) {
-            if (c->graph_exec) {
-                (void)hipGraphExecDestroy(c->graph_exec);
-                c->graph_exec = nullptr;
-            }
+            drop_graph(c);
             // plan buffers must exist before capture (no allocation inside)
-            if (l.chunk > 0 && c->plan.chunk != l.chunk) {
-                int rc = build_plan(c, l.chunk);
-                if (rc != SMOC_OK)
-                    return rc;
-            }
-            if (l.persistent && !c->queue.p) {
-                int rc = c->queue.alloc(2);
-                if (rc != SMOC_OK)
-                    return rc;
-            }
+            rc = ctx_bufs(c, l, &b);
+            if (rc != SMOC_OK)
+                return rc;
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_run(c, l, nullptr);
+            rc = enqueue_run(c, l, b, nullptr);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc != SMOC_OK) {
                 if (g)
@@ -1073,20 +1177,10 @@ static int run_once(smoc_ctx *c, const smoc_launch &l, smoc_timing *tm)
         tm->sweep_ms = ms;  // one graph: phases are not timed separately
         return SMOC_OK;
     }
-    int rc = enqueue_run(c, l, c->ev);
+    rc = ctx_bufs(c, l, &b);
     if (rc != SMOC_OK)
         return rc;
-    HIPCHK(hipEventSynchronize(c->ev[4]));
-    float ms[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int i = 0; i < 4; i++)
-        HIPCHK(hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]));
-    *tm = smoc_timing{};
-    tm->reset_ms = ms[0];
-    tm->xfer_ms = ms[1];
-    tm->sweep_ms = ms[2];
-    tm->reduce_ms = ms[3];
-    tm->total_ms = (double)ms[0] + ms[1] + ms[2] + ms[3];
-    return SMOC_OK;
+    return run_eager(c, l, b, tm);
 }
 
 // Measures candidate launch shapes on the loaded problem (each run is a
@@ -1186,6 +1280,33 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
     return SMOC_OK;
 }
 
+// After a completed run: the FIXED64 range report, the results flag and the
+// caller's timing record (n_items: work entries before the group slices).
+static int finish_run(smoc_ctx *c, const smoc_launch &l, const smoc_timing &tm, int64_t n_items,
+                      smoc_timing *timing, const char *what)
+{
+    if (c->fixed()) {
+        uint32_t overflows = 0;
+        HIPCHK(hipMemcpy(&overflows, c->flux.p + 248, sizeof(overflows), hipMemcpyDeviceToHost));
+        if (overflows) {
+            smoc_set_error("%s: %u FIXED64 run sums exceeded the per-contribution bound "
+                           "and were dropped; fine_flux is incomplete (use an FP64 tally)",
+                           what, overflows);
+            return SMOC_ERR_RANGE;
+        }
+    }
+    c->ran = true;
+    if (timing) {
+        const int G = c->p.egroups;
+        const int slices = (G + 64 * l.groups_per_lane - 1) / (64 * l.groups_per_lane);
+        *timing = tm;
+        timing->segments = c->S;
+        timing->work_items = n_items * slices;
+        timing->used = l;
+    }
+    return SMOC_OK;
+}
+
 int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
 {
     if (!c) {
@@ -1215,26 +1336,86 @@ int smoc_run(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
     int rc = run_once(c, l, &tm);
     if (rc != SMOC_OK)
         return rc;
-    if (c->fixed()) {
-        uint32_t overflows = 0;
-        HIPCHK(hipMemcpy(&overflows, c->flux.p + 248, sizeof(overflows), hipMemcpyDeviceToHost));
-        if (overflows) {
-            smoc_set_error("smoc_run: %u FIXED64 run sums exceeded the per-contribution bound "
-                           "and were dropped; fine_flux is incomplete (use an FP64 tally)",
-                           overflows);
-            return SMOC_ERR_RANGE;
+    return finish_run(c, l, tm, l.chunk > 0 ? c->plan.n_items : c->T, timing, "smoc_run");
+}
+
+int smoc_run_noalloc(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing)
+{
+    const char *what = "smoc_run_noalloc";
+    if (!c) {
+        smoc_set_error("%s: NULL context", what);
+        return SMOC_ERR_ARG;
+    }
+    if (!c->regions_set || !c->segments_set) {
+        smoc_set_error("%s: call smoc_set_regions and smoc_set_segments first", what);
+        return SMOC_ERR_STATE;
+    }
+    smoc_launch l;
+    fill_launch(c, launch, &l);
+    if (l.autotune || l.use_graph) {
+        smoc_set_error("%s: autotune and use_graph allocate; pass them as 0", what);
+        return SMOC_ERR_ARG;
+    }
+    int rc = check_launch(c, l, what);
+    if (rc != SMOC_OK)
+        return rc;
+    HIPCHK(hipSetDevice(c->p.device));
+    unsigned char *base = scratch_of(c->p.device);
+    if (!base) {
+        smoc_set_error("%s: no scratch address for device %d", what, c->p.device);
+        return SMOC_ERR_STATE;
+    }
+    // layout: work counters, then the chunk plan (items, the two work lists,
+    // track_item_off, chunk_psi, xfer), each 256-byte aligned; sizes from the
+    // host copy of the track offsets, as the device plan will make them
+    RunBufs b;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + off;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    b.queue = reinterpret_cast<int64_t *>(take(2 * sizeof(int64_t)));
+    if (l.chunk > 0) {
+        const int64_t C = l.chunk;
+        int64_t n = 0, n1 = 0, n2 = 0;
+        for (int64_t t = 0; t < c->T; t++) {
+            const int64_t len = c->h_track_off[t + 1] - c->h_track_off[t];
+            const int64_t nc = len > C ? (len + C - 1) / C : 1;
+            n += nc;
+            n1 += nc > 1 ? nc - 1 : 1;
+            n2 += nc - 1;
         }
+        const size_t gp = (size_t)c->Gp;
+        const size_t need = 256 + ((size_t)n * 32 + 255) / 256 * 256 +
+                            ((size_t)std::max<int64_t>(n1, 1) * 8 + 255) / 256 * 256 +
+                            ((size_t)std::max<int64_t>(n2, 1) * 8 + 255) / 256 * 256 +
+                            ((size_t)(c->T + 1) * 8 + 255) / 256 * 256 +
+                            ((size_t)n * gp * 4 + 255) / 256 * 256 +
+                            ((size_t)n * gp * 8 + 255) / 256 * 256;
+        if (need > (size_t)SMOC_NOALLOC_SCRATCH_BYTES) {
+            smoc_set_error("%s: the plan of %lld chunks of %d segments needs %zu bytes of "
+                           "scratch, more than its %zu (use longer chunks or smoc_run)",
+                           what, (long long)n, l.chunk, need,
+                           (size_t)SMOC_NOALLOC_SCRATCH_BYTES);
+            return SMOC_ERR_ARG;
+        }
+        b.items = reinterpret_cast<int64_t *>(take((size_t)n * 32));
+        b.list1 = reinterpret_cast<int64_t *>(take((size_t)std::max<int64_t>(n1, 1) * 8));
+        b.list2 = reinterpret_cast<int64_t *>(take((size_t)std::max<int64_t>(n2, 1) * 8));
+        b.track_item_off = reinterpret_cast<int64_t *>(take((size_t)(c->T + 1) * 8));
+        b.chunk_psi = reinterpret_cast<float *>(take((size_t)n * gp * 4));
+        b.xfer = reinterpret_cast<float *>(take((size_t)n * gp * 8));
+        b.n_items = n;
+        b.n1 = n1;
+        b.n2 = n2;
+        b.plan_chunk = l.chunk;  // rebuilt in the scratch by this run
     }
-    c->ran = true;
-    if (timing) {
-        const int G = c->p.egroups;
-        const int slices = (G + 64 * l.groups_per_lane - 1) / (64 * l.groups_per_lane);
-        *timing = tm;
-        timing->segments = c->S;
-        timing->work_items = (l.chunk > 0 ? c->plan.n_items : c->T) * slices;
-        timing->used = l;
-    }
-    return SMOC_OK;
+    smoc_timing tm{};
+    rc = run_eager(c, l, b, &tm);
+    if (rc != SMOC_OK)
+        return rc;
+    return finish_run(c, l, tm, l.chunk > 0 ? b.n_items : c->T, timing, what);
 }
 
 int smoc_get_results(smoc_ctx *c, double *fine_flux, float *psi_out)

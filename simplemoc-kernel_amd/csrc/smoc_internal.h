@@ -116,6 +116,11 @@ int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
 int64_t smoc_update_blocks(int64_t n);
 int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *stream);
 int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
+/* chunk plan of a chunked sweep on the device (k_plan): items [n][4], the
+ * phase-1 / phase-2 work lists and track_item_off [T+1], for chunk length
+ * chunk >= 1; the sizes follow from track_off as for the host plan */
+int smoc_launch_plan(const int64_t *track_off, int64_t T, int chunk, int64_t *items,
+                     int64_t *list1, int64_t *list2, int64_t *track_item_off, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 /* plain copy of n16 16-byte elements, kernel variant 0..SMOC_COPY_VARIANTS-1 */
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int variant, int cus,

@@ -968,6 +968,76 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
     }
 }
 
+// ---- chunk plan built on the device ---------------------------------------------
+// The work items of a chunked sweep, written without host memory (the
+// allocation-free entry point builds them in its scratch on every call).  Track
+// t becomes nc = max(1, ceil(len / C)) chunks, items [i][4] = (track, begin,
+// end, first run id = begin); list1 holds each track's first chunk (ITEM_EMIT
+// when it has more) and its middle chunks (ITEM_XFER), list2 every chunk after
+// the first, tio[t] the first chunk of track t and tio[T] the chunk count.
+// These are the items of the host plan; the lists are in track order.  One
+// workgroup of 1024 threads: thread i takes a contiguous range of tracks,
+// counts its chunks and list entries, a scan over the threads gives its
+// offsets, then it writes.
+__global__ void __launch_bounds__(1024)
+    k_plan(const int64_t *__restrict__ track_off, int64_t T, int C, int64_t *__restrict__ items,
+           int64_t *__restrict__ list1, int64_t *__restrict__ list2, int64_t *__restrict__ tio)
+{
+    __shared__ int64_t s_n[1024], s_1[1024], s_2[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (T + 1023) / 1024;
+    const int64_t t0 = min(T, (int64_t)tid * per), t1 = min(T, t0 + per);
+    auto chunks = [&](int64_t t) {
+        const int64_t len = track_off[t + 1] - track_off[t];
+        return len > C ? (len + C - 1) / C : (int64_t)1;
+    };
+    int64_t n = 0, n1 = 0, n2 = 0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t nc = chunks(t);
+        n += nc;
+        n1 += nc > 1 ? nc - 1 : 1;
+        n2 += nc - 1;
+    }
+    s_n[tid] = n;
+    s_1[tid] = n1;
+    s_2[tid] = n2;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan over the threads
+        int64_t a = 0, b = 0, c = 0;
+        if (tid >= d) {
+            a = s_n[tid - d];
+            b = s_1[tid - d];
+            c = s_2[tid - d];
+        }
+        __syncthreads();
+        s_n[tid] += a;
+        s_1[tid] += b;
+        s_2[tid] += c;
+        __syncthreads();
+    }
+    int64_t i = s_n[tid] - n, p1 = s_1[tid] - n1, p2 = s_2[tid] - n2;
+    for (int64_t t = t0; t < t1; ++t) {
+        const int64_t b = track_off[t], e = track_off[t + 1], nc = chunks(t);
+        tio[t] = i;
+        for (int64_t k = 0; k < nc; ++k) {
+            const int64_t cb = b + k * C, ce = cb + C < e ? cb + C : e;
+            int64_t *it = items + 4 * (i + k);
+            it[0] = t;
+            it[1] = cb;
+            it[2] = ce;
+            it[3] = cb;  // first run id: every run is one segment (chunking is refused otherwise)
+        }
+        list1[p1++] = (i << 2) | (nc > 1 ? ITEM_EMIT : 0);
+        for (int64_t k = 1; k + 1 < nc; ++k)
+            list1[p1++] = ((i + k) << 2) | ITEM_XFER;
+        for (int64_t k = 1; k < nc; ++k)
+            list2[p2++] = (i + k) << 2;
+        i += nc;
+    }
+    if (tid == 1023)  // the last thread's range ends at T: i is the chunk count
+        tio[T] = i;
+}
+
 // ---- packing of region data (once per smoc_set_regions) ------------------------
 template <int STORE>
 __global__ void __launch_bounds__(256) k_pack(smoc_pack_args a)
@@ -1459,6 +1529,17 @@ extern "C" int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F,
         return 0;
     hipLaunchKernelGGL(k_encode_stream, dim3(blocks_for(S, 256)), dim3(256), 0,
                        (hipStream_t)stream, zin_io, code_io, F, S);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_plan(const int64_t *track_off, int64_t T, int chunk, int64_t *items,
+                                int64_t *list1, int64_t *list2, int64_t *track_item_off,
+                                void *stream)
+{
+    if (T <= 0 || chunk <= 0)
+        return T < 0 || chunk <= 0 ? (int)hipErrorInvalidValue : 0;
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                       track_off, T, chunk, items, list1, list2, track_item_off);
     return (int)hipGetLastError();
 }
 

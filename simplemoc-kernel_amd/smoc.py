@@ -139,6 +139,7 @@ SIGNATURES = {
     "smoc_set_regions": (_i, [_P, _F32, _F32, _P]),
     "smoc_set_segments": (_i, [_P, _i64, _F32, _I32, _F32, _i64, _I64, _F32]),
     "smoc_run": (_i, [_P, ctypes.POINTER(Launch), ctypes.POINTER(Timing)]),
+    "smoc_run_noalloc": (_i, [_P, ctypes.POINTER(Launch), ctypes.POINTER(Timing)]),
     "smoc_get_results": (_i, [_P, _P, _P]),
     "smoc_trace_count": (_i, [_i, ctypes.POINTER(Geometry), _u64, _i64, _I64,
                               ctypes.POINTER(_i64)]),
@@ -362,6 +363,15 @@ class Context:
         ln = Launch(**launch)
         tm = Timing()
         _check(lib().smoc_run(self._h, ctypes.byref(ln), ctypes.byref(tm)), "smoc_run")
+        return tm
+
+    def run_noalloc(self, **launch) -> Timing:
+        """smoc_run_noalloc: the sweep with its scratch in the library's fixed
+        per-GPU array (no allocation in the call)."""
+        ln = Launch(**launch)
+        tm = Timing()
+        _check(lib().smoc_run_noalloc(self._h, ctypes.byref(ln), ctypes.byref(tm)),
+               "smoc_run_noalloc")
         return tm
 
     def results(self):
