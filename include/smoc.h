@@ -76,14 +76,18 @@ extern "C" {
  *               a fixed order.  Bit-exact run to run and across launch
  *               shapes; chunked sweeps of a region-sorted stream are refused
  *               (SMOC_ERR_ARG), since chunks would split runs.
- *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums.
+ *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums (of the
+ *               FP32 block sums, on a region-sorted stream).
  *   FIXED64     deterministic: each run is rounded to a 64-bit fixed-point
  *               integer (2^-32 resolution, |fine_flux| < 2^31) and added with
  *               integer atomics, which are associative, so the result is
  *               bit-exact run to run in any order; converted to FP64 after
- *               the sweep.  smoc_run fails with SMOC_ERR_RANGE if a value
- *               left the range.  Chunked sweeps of a region-sorted stream
- *               are refused as for DET_F64. */
+ *               the sweep.  Every contribution (on a region-sorted stream,
+ *               every FP32 sum of a block of up to 8 segments, against 8x the
+ *               bound) is checked against a per-contribution bound that keeps
+ *               the sums in range; smoc_run fails with SMOC_ERR_RANGE if one
+ *               exceeded it.  Chunked sweeps of a region-sorted stream are
+ *               refused as for DET_F64. */
 #define SMOC_TALLY_ATOMIC_F32 0
 #define SMOC_TALLY_ATOMIC_F64 1
 #define SMOC_TALLY_DET_F64 2

@@ -159,7 +159,7 @@ __device__ __forceinline__ void fit_coeffs(const float4 t, int kind, float &c0, 
             c2 = 0.f;
         } else {
             c1 = (t.y - t.w) * kInv2Dz;
-            c2 = ((t.y - 2.f * t.z) + t.w) * kInv2Dz2;
+            c2 = (__builtin_fmaf(-2.f, t.z, t.y) + t.w) * kInv2Dz2;
         }
     }
 }
@@ -199,10 +199,10 @@ struct EvTerms {
 __device__ __forceinline__ float r4_series(float tau)
 {
     float r4 = 0x1.3cd4c4p-16f;
-    r4 = r4 * tau - 0x1.95bd10p-13f;
-    r4 = r4 * tau + 0x1.6ba232p-10f;
-    r4 = r4 * tau - 0x1.110f3ep-7f;
-    return r4 * tau + 0x1.555552p-5f;
+    r4 = __builtin_fmaf(r4, tau, -0x1.95bd10p-13f);
+    r4 = __builtin_fmaf(r4, tau, 0x1.6ba232p-10f);
+    r4 = __builtin_fmaf(r4, tau, -0x1.110f3ep-7f);
+    return __builtin_fmaf(r4, tau, 0x1.555552p-5f);
 }
 
 __device__ __forceinline__ EvTerms ev_series(float tau)
@@ -362,24 +362,20 @@ __device__ __forceinline__ void acc_init(Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, f
     acc.lim = lim / SMOC_WEIGHT;
 }
 
-// Block sums (region-sorted stream, FP64 and FIXED64 tallies): each segment's
-// flux integral is checked as a contribution (FIXED64's bound, nothing for
-// the others), and the block's FP32 sum joins the run sum with one add.
-template <typename A> __device__ __forceinline__ bool acc_fits(const A &, float) { return true; }
-template <bool SINGLE>
-__device__ __forceinline__ bool acc_fits(const Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, float x)
-{
-    return fabsf(x) <= acc.lim;
-}
-template <typename A> __device__ __forceinline__ void acc_add_block(A &acc, float sum, bool)
+// Block sums (region-sorted stream): the flux integrals of a block of up to 8
+// segments of a run are summed in FP32, and the block sum joins the run's
+// accumulator with one add (compensated for DET_KAHAN).  FIXED64 checks the
+// block sum against n times its per-contribution bound: a block of n in-bound
+// contributions passes, so no run of in-bound contributions can wrap.
+template <typename A> __device__ __forceinline__ void acc_add_block(A &acc, float sum, int)
 {
     acc.add(sum);
 }
 template <bool SINGLE>
 __device__ __forceinline__ void acc_add_block(Acc<SMOC_TALLY_FIXED64, SINGLE> &acc, float sum,
-                                              bool fits)
+                                              int n)
 {
-    acc.bad |= !fits;
+    acc.bad |= !(fabsf(sum) <= (float)n * acc.lim);
     acc.s += (double)sum;
 }
 
@@ -430,26 +426,29 @@ enum { ITEM_XFER = 1, ITEM_EMIT = 2 };
 // region tuple only.  With them a segment costs tau, ev and the two affine
 // updates.
 struct RunK {
-    float sig, i1, i2, i2x2;  // sigT, 1/sigT, 1/sigT^2, 2/sigT^2
-    float c0, c1, c2;         // source fit: q0 = c0 + c1 zin + c2 zin^2
-    float a1, b1;             // mu q1 = a1 + b1 zin
-    float q2m, q2m2;          // mu2 q2 and 2 mu2 q2: the integral's q2 term is
-                              // (2 mu2 q2 / sigT^4) poly / 6 = 2 mu2 q2 ds^4 r4
+    float sig, i1, i2x2;      // sigT, 1/sigT, 2/sigT^2
+    float c0, c1, c2;         // source fit: q0 = c0 + zin (c1 + c2 zin)
+    float na1;                // -mu c1: mu q1 = mu (c1 + 2 c2 zin) = 2 mu (c1 + c2 zin) - mu c1
+    float q2m;                // mu2 q2: the integral's q2 term is
+                              // (2 mu2 q2 / sigT^4) poly / 6 = mu2 q2 (2 ds^4) r4
 };
 
+// The run loop's arithmetic (run_consts, attenuate_run, series_a, series_b and
+// the per-segment powers) is written with explicit fused multiply-adds and no
+// contraction, so every instantiation -- 1 or 2 groups per lane, any exp or
+// tally variant -- performs the same operations in the same order and the
+// deterministic tallies stay bit-exact across launch shapes.
 template <int STORE>
 __device__ __forceinline__ RunK run_consts(const float4 t, int kind)
 {
+#pragma clang fp contract(off)
     RunK k;
     k.sig = t.x;
     k.i1 = rcp(t.x);
-    k.i2 = k.i1 * k.i1;
-    k.i2x2 = 2.f * k.i2;
+    k.i2x2 = 2.f * (k.i1 * k.i1);
     fit_coeffs<STORE>(t, kind, k.c0, k.c1, k.c2);
-    k.a1 = SMOC_MU * k.c1;
-    k.b1 = (2.f * SMOC_MU) * k.c2;
+    k.na1 = -SMOC_MU * k.c1;
     k.q2m = SMOC_MU2 * k.c2;
-    k.q2m2 = 2.f * k.q2m;
     return k;
 }
 
@@ -459,17 +458,21 @@ template <int EXPM>
 __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zin, float zin2,
                                                float &psi, float &one_m_ev, const ExpTab &tb)
 {
+#pragma clang fp contract(off)
     const float tau = k.sig * ds;
     const EvTerms x = ev_terms<EXPM>(tau, tb);
     const float e1 = x.ev * k.i1;
-    const float m2 = x.tme * k.i2;
-    const float reuse = tau * (tau - 2.f) + e1 * k.i2x2;
-    const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
-    const float q1m = k.a1 + k.b1 * zin;
-    const float kfi6 = k.q2m2 * (k.i2 * k.i2);  // 2 mu2 q2 / sigT^4
+    const float i2 = 0.5f * k.i2x2;  // 1/sigT^2 (this path is rare on the benchmark streams)
+    const float m2 = x.tme * i2;
+    const float reuse = __builtin_fmaf(e1, k.i2x2, tau * (tau - 2.f));
+    const float q0 = __builtin_fmaf(k.c2, zin2, __builtin_fmaf(k.c1, zin, k.c0));
+    const float q1m = SMOC_MU * __builtin_fmaf(2.f * k.c2, zin, k.c1);
+    const float kfi6 = (2.f * k.q2m) * (i2 * i2);  // 2 mu2 q2 / sigT^4
     const float flux_integral =
-        q0 * m2 + psi * e1 + q1m * reuse + kfi6 * (x.poly * (1.f / 6.f));
-    psi = q0 * e1 + q1m * m2 + k.q2m * reuse + psi * x.one_m_ev;
+        __builtin_fmaf(q0, m2, __builtin_fmaf(psi, e1, __builtin_fmaf(q1m, reuse,
+                                                          kfi6 * (x.poly * (1.f / 6.f)))));
+    psi = __builtin_fmaf(q0, e1, __builtin_fmaf(q1m, m2, __builtin_fmaf(k.q2m, reuse,
+                                                                        psi * x.one_m_ev)));
     one_m_ev = x.one_m_ev;
     return flux_integral;
 }
@@ -484,35 +487,41 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 // zero and skipped, with the same results.
 struct SegA {
     float e1, ev;   // ev / sigT, 1 - e^-tau
-    float fpart;    // the flux integral without its psi term
     float src;      // the source part S of psi' = S + psi (1 - ev)
 };
 
-// ds2 = ds^2 and ds4 = ds^4 are the segment's (shared by every group):
+// ds2 = ds^2 and ds4x2 = 2 ds^4 are the segment's (shared by every group):
 // tau - ev = tau^2 r, so (tau - ev) / sigT^2 = ds^2 r, and the q2 term of
-// the integral, (2 mu2 q2 / sigT^4) tau^4 r4, is 2 mu2 q2 ds^4 r4.
+// the integral, (2 mu2 q2 / sigT^4) tau^4 r4, is mu2 q2 (2 ds^4) r4.  The
+// psi-free part of the flux integral is added into the block sum bsum at the
+// end of its chain of fused multiply-adds, so it costs no add of its own and
+// no register per segment.
 template <bool HAS_C2>
-__device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, float ds4, float zin,
-                                         float zin2)
+__device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, float ds4x2, float zin,
+                                         float &bsum)
 {
+#pragma clang fp contract(off)
     const float tau = k.sig * ds;
     const float r4 = r4_series(tau);
     const float t2 = tau * tau;
-    const float r = (0.5f - tau * (1.f / 6.f)) + t2 * r4;
+    const float r = __builtin_fmaf(t2, r4, __builtin_fmaf(tau, -1.f / 6.f, 0.5f));
     SegA a;
-    a.ev = tau - t2 * r;  // one fused rounding
+    a.ev = __builtin_fmaf(-t2, r, tau);  // one fused rounding
     a.e1 = a.ev * k.i1;
     const float m2 = ds2 * r;  // (tau - ev) / sigT^2
-    const float reuse = (t2 - 2.f * tau) + a.e1 * k.i2x2;
+    const float reuse = __builtin_fmaf(a.e1, k.i2x2, __builtin_fmaf(-2.f, tau, t2));
     if constexpr (HAS_C2) {
-        const float q0 = k.c0 + k.c1 * zin + k.c2 * zin2;
-        const float q1m = k.a1 + k.b1 * zin;
-        a.fpart = q0 * m2 + q1m * reuse + (k.q2m2 * ds4) * r4;
-        a.src = q0 * a.e1 + q1m * m2 + k.q2m * reuse;
+        // Horner in zin: no zin^2 per segment; mu q1 from the same inner term
+        const float inner = __builtin_fmaf(k.c2, zin, k.c1);
+        const float q0 = __builtin_fmaf(inner, zin, k.c0);
+        const float q1m = __builtin_fmaf(inner, 2.f * SMOC_MU, k.na1);
+        bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(q1m, reuse,
+                                                     __builtin_fmaf(k.q2m * ds4x2, r4, bsum)));
+        a.src = __builtin_fmaf(q0, a.e1, __builtin_fmaf(q1m, m2, k.q2m * reuse));
     } else {
-        const float q0 = k.c0 + k.c1 * zin;
-        a.fpart = q0 * m2 + k.a1 * reuse;
-        a.src = q0 * a.e1 + k.a1 * m2;
+        const float q0 = __builtin_fmaf(k.c1, zin, k.c0);
+        bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(-k.na1, reuse, bsum));
+        a.src = __builtin_fmaf(q0, a.e1, -k.na1 * m2);
     }
     return a;
 }
@@ -520,12 +529,13 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
 // psi' = S + psi (1 - ev) as psi + (S - psi ev): the small change is formed
 // first (one FMA), then added to psi, so only one rounding happens at the
 // magnitude of psi (a run near its steady state psi = S / ev would otherwise
-// accumulate the roundings of every term).  Returns the flux integral.
-__device__ __forceinline__ float series_b(const SegA &a, float &psi)
+// accumulate the roundings of every term).  The psi term of the flux
+// integral, psi ev / sigT, joins the block sum.
+__device__ __forceinline__ void series_b(const SegA &a, float &psi, float &bsum)
 {
-    const float flux_integral = a.fpart + psi * a.e1;
-    psi = psi + (a.src - psi * a.ev);
-    return flux_integral;
+#pragma clang fp contract(off)
+    bsum = __builtin_fmaf(psi, a.e1, bsum);
+    psi = psi + __builtin_fmaf(-psi, a.ev, a.src);
 }
 
 template <int STORE, int GPL, int EXPM, int TALLY, bool RUNS, int MODE>
@@ -700,25 +710,11 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     auto series_ok = [&](const RawSeg &sg) {
                         return EXPM == SMOC_EXP_ACCURATE && __float_as_uint(sg.len) < thr;
                     };
-                    // blocks of up to NB segments: 8, or 4 for FIXED64, whose
-                    // blocks of 8 need more than the 128 VGPRs a wave may hold
-                    // at 16 waves per workgroup with 2 groups per lane (the
-                    // same 4 at 1 group per lane keeps the block sums, and so
-                    // the results, identical across launch shapes); 1 where
-                    // the exp mode has no series branch (one segment a step)
-                    constexpr int NB = EXPM != SMOC_EXP_ACCURATE          ? 1
-                                       : TALLY == SMOC_TALLY_FIXED64 ? 4
-                                                                     : kSeries;
-                    constexpr bool kBlockSum = MODE == MODE_SWEEP &&
-                                               (TALLY == SMOC_TALLY_DET_F64 ||
-                                                TALLY == SMOC_TALLY_ATOMIC_F64 ||
-                                                TALLY == SMOC_TALLY_FIXED64);
-                    auto tally = [&](int j, float fi, float ev) {
-                        if constexpr (MODE == MODE_XFER)
-                            mprod[j] *= 1.f - ev;
-                        else
-                            acc[j].add(fi);
-                    };
+                    // blocks of up to NB = 8 segments (the same length for
+                    // every tally and launch shape, so the block sums and the
+                    // results do not depend on the shape); 1 where the exp
+                    // mode has no series branch (one segment a step)
+                    constexpr int NB = EXPM != SMOC_EXP_ACCURATE ? 1 : kSeries;
                     for (;;) {
                         // the next NB segments, from one 64-bit base per array
                         // so that the scalar loads take constant offsets
@@ -759,36 +755,32 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
+                                float bs[GPL];
+#pragma unroll
+                                for (int j = 0; j < GPL; j++)
+                                    bs[j] = 0.f;
 #pragma unroll
                                 for (int u = 0; u < N; u++) {
                                     const RawSeg &sg = u == 0 ? cur : nx[u - 1];
-                                    const float z2 = HAS_C2 ? sg.zin * sg.zin : 0.f;
                                     const float d2 = sg.len * sg.len;
-                                    const float d4 = HAS_C2 ? d2 * d2 : 0.f;
+                                    const float d4x2 = HAS_C2 ? (2.f * d2) * d2 : 0.f;
 #pragma unroll
                                     for (int j = 0; j < GPL; j++)
-                                        a[u][j] = series_a<HAS_C2>(rk[j], sg.len, d2, d4, sg.zin,
-                                                                   z2);
+                                        a[u][j] = series_a<HAS_C2>(rk[j], sg.len, d2, d4x2, sg.zin,
+                                                                   bs[j]);
                                 }
 #pragma unroll
                                 for (int j = 0; j < GPL; j++) {
-                                    if constexpr (kBlockSum) {
-                                        // the block's flux integrals summed in
-                                        // FP32, then once into the FP64 run sum
-                                        float bs = series_b(a[0][j], psi[j]);
-                                        bool fits = acc_fits(acc[j], bs);
 #pragma unroll
-                                        for (int u = 1; u < N; u++) {
-                                            const float fi = series_b(a[u][j], psi[j]);
-                                            fits &= acc_fits(acc[j], fi);
-                                            bs += fi;
-                                        }
-                                        acc_add_block(acc[j], bs, fits);
-                                    } else {
-#pragma unroll
-                                        for (int u = 0; u < N; u++)
-                                            tally(j, series_b(a[u][j], psi[j]), a[u][j].ev);
+                                    for (int u = 0; u < N; u++) {
+                                        if constexpr (MODE == MODE_XFER)
+                                            mprod[j] *= 1.f - a[u][j].ev;
+                                        series_b(a[u][j], psi[j], bs[j]);
                                     }
+                                    // the block's flux integrals, summed in FP32, join
+                                    // the run's accumulator once
+                                    if constexpr (MODE != MODE_XFER)
+                                        acc_add_block(acc[j], bs[j], N);
                                 }
                                 k += N;
                                 cur = nx[N - 1];
@@ -798,8 +790,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                 block(std::integral_constant<int, (NB >= 8 ? 8 : 1)>{});
                             else if (NB >= 4 && n >= 4)
                                 block(std::integral_constant<int, (NB >= 4 ? 4 : 1)>{});
-                            else if (n >= 2)
-                                block(std::integral_constant<int, 2>{});
+                            else if (NB >= 2 && n >= 2)
+                                block(std::integral_constant<int, (NB >= 2 ? 2 : 1)>{});
                             else
                                 block(std::integral_constant<int, 1>{});
                         } else {

@@ -247,9 +247,11 @@ def test_fixed64_sorted_and_init(gpu_ready):
     check_vs_oracle(prob, flux, psi, storage="f16", init=init)
 
 
-def test_fixed64_out_of_range_is_reported(gpu_ready):
-    # sigT = 1e-4 makes the 2 ev / sigT^3 term of the flux integral exceed 2^31
-    prob = smoc.generate(2_000, G=64, R=20, F=5, seed=4)
+@pytest.mark.parametrize("sorted_stream", [False, True])
+def test_fixed64_out_of_range_is_reported(gpu_ready, sorted_stream):
+    # sigT = 1e-4 makes the 2 ev / sigT^3 term of the flux integral exceed 2^31;
+    # on a sorted stream the bound is checked on the run loop's block sums
+    prob = smoc.generate(2_000, G=64, R=20, F=5, seed=4, sorted_stream=sorted_stream)
     prob.sigT = np.full_like(prob.sigT, 1e-4)
     prob.len = np.full_like(prob.len, 0.1)
     with smoc.Context(64, 20, 5, tally="fixed64") as ctx:
