@@ -1090,6 +1090,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
         r.init = c->flux_init.p;
         r.flux = c->flux.p + 256;
         r.n_fsr = nrow;
+        r.G = G;
         r.Gp = c->Gp;
         r.slices = slices;
         r.single = c->runs ? 0 : 1;

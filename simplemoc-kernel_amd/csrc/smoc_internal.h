@@ -85,7 +85,7 @@ typedef struct {
     const float *init;           /* fine_flux init [R*F][Gp] or NULL */
     void *flux;                  /* out fine_flux [R*F][Gp]: f64 (DET_F64) / f32 (DET_KAHAN) */
     int64_t n_fsr;
-    int Gp, slices;
+    int G, Gp, slices;           /* lanes of groups >= G (row padding) are skipped */
     int single;                  /* partials are single FP32 tallies */
 } smoc_reduce_args;
 

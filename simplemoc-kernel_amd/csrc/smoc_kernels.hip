@@ -1276,6 +1276,8 @@ __global__ void __launch_bounds__(64) k_zero_i64(int64_t *p, int64_t n)
 // ---- deterministic segmented reduction over run partials -------------------------
 // One wave per ((region, interval), group slice); the runs of a (region,
 // interval) are summed in ascending run-id order, whatever the launch shape.
+// Lanes of the row padding (groups >= G) neither read partials nor write: no
+// sweep writes their partials and no reader uses their fine_flux.
 template <int TALLY, int GPL, bool SINGLE>
 __global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
 {
@@ -1293,6 +1295,8 @@ __global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         const int g = sl * 64 * GPL + j * 64 + lane;
+        if (g >= a.G)
+            continue;
         const int64_t o = i * Gp + g;
         if constexpr (TALLY == SMOC_TALLY_DET_F64) {
             using part_t = typename std::conditional<SINGLE, float, double>::type;
