@@ -486,8 +486,8 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
 // fit is linear (c2 = 0, so q2 = 0 and q1 = c1): the terms in c2 are exactly
 // zero and skipped, with the same results.
 struct SegA {
-    float e1, ev;   // ev / sigT, 1 - e^-tau
-    float src;      // the source part S of psi' = S + psi (1 - ev)
+    float ev;   // 1 - e^-tau
+    float src;  // the source part S of psi' = S + psi (1 - ev)
 };
 
 // ds2 = ds^2 and ds4x2 = 2 ds^4 are the segment's (shared by every group):
@@ -507,9 +507,9 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
     const float r = __builtin_fmaf(t2, r4, __builtin_fmaf(tau, -1.f / 6.f, 0.5f));
     SegA a;
     a.ev = __builtin_fmaf(-t2, r, tau);  // one fused rounding
-    a.e1 = a.ev * k.i1;
-    const float m2 = ds2 * r;  // (tau - ev) / sigT^2
-    const float reuse = __builtin_fmaf(a.e1, k.i2x2, __builtin_fmaf(-2.f, tau, t2));
+    const float e1 = a.ev * k.i1;  // ev / sigT
+    const float m2 = ds2 * r;      // (tau - ev) / sigT^2
+    const float reuse = __builtin_fmaf(e1, k.i2x2, __builtin_fmaf(-2.f, tau, t2));
     if constexpr (HAS_C2) {
         // Horner in zin: no zin^2 per segment; mu q1 from the same inner term
         const float inner = __builtin_fmaf(k.c2, zin, k.c1);
@@ -517,11 +517,11 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
         const float q1m = __builtin_fmaf(inner, 2.f * SMOC_MU, k.na1);
         bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(q1m, reuse,
                                                      __builtin_fmaf(k.q2m * ds4x2, r4, bsum)));
-        a.src = __builtin_fmaf(q0, a.e1, __builtin_fmaf(q1m, m2, k.q2m * reuse));
+        a.src = __builtin_fmaf(q0, e1, __builtin_fmaf(q1m, m2, k.q2m * reuse));
     } else {
         const float q0 = __builtin_fmaf(k.c1, zin, k.c0);
         bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(-k.na1, reuse, bsum));
-        a.src = __builtin_fmaf(q0, a.e1, -k.na1 * m2);
+        a.src = __builtin_fmaf(q0, e1, -k.na1 * m2);
     }
     return a;
 }
@@ -530,11 +530,13 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
 // first (one FMA), then added to psi, so only one rounding happens at the
 // magnitude of psi (a run near its steady state psi = S / ev would otherwise
 // accumulate the roundings of every term).  The psi term of the flux
-// integral, psi ev / sigT, joins the block sum.
-__device__ __forceinline__ void series_b(const SegA &a, float &psi, float &bsum)
+// integral, psi ev / sigT, is summed as psi ev into psum; the block multiplies
+// that sum by 1 / sigT once, so a segment keeps two values (ev, S) instead of
+// three between series_a and series_b.
+__device__ __forceinline__ void series_b(const SegA &a, float &psi, float &psum)
 {
 #pragma clang fp contract(off)
-    bsum = __builtin_fmaf(psi, a.e1, bsum);
+    psum = __builtin_fmaf(psi, a.ev, psum);
     psi = psi + __builtin_fmaf(-psi, a.ev, a.src);
 }
 
@@ -755,10 +757,10 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
-                                float bs[GPL];
+                                float bs[GPL], ps[GPL];  // psi-free part; sum of psi ev
 #pragma unroll
                                 for (int j = 0; j < GPL; j++)
-                                    bs[j] = 0.f;
+                                    bs[j] = ps[j] = 0.f;
 #pragma unroll
                                 for (int u = 0; u < N; u++) {
                                     const RawSeg &sg = u == 0 ? cur : nx[u - 1];
@@ -775,12 +777,13 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                     for (int u = 0; u < N; u++) {
                                         if constexpr (MODE == MODE_XFER)
                                             mprod[j] *= 1.f - a[u][j].ev;
-                                        series_b(a[u][j], psi[j], bs[j]);
+                                        series_b(a[u][j], psi[j], ps[j]);
                                     }
                                     // the block's flux integrals, summed in FP32, join
                                     // the run's accumulator once
                                     if constexpr (MODE != MODE_XFER)
-                                        acc_add_block(acc[j], bs[j], N);
+                                        acc_add_block(acc[j],
+                                                      __builtin_fmaf(ps[j], rk[j].i1, bs[j]), N);
                                 }
                                 k += N;
                                 cur = nx[N - 1];
