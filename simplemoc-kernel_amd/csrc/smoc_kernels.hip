@@ -426,9 +426,10 @@ enum { ITEM_XFER = 1, ITEM_EMIT = 2 };
 // region tuple only.  With them a segment costs tau, ev and the two affine
 // updates.
 struct RunK {
-    float sig, i1, i2x2;      // sigT, 1/sigT, 2/sigT^2
-    float c0, c1, c2;         // source fit: q0 = c0 + zin (c1 + c2 zin)
-    float na1;                // -mu c1: mu q1 = mu (c1 + 2 c2 zin) = 2 mu (c1 + c2 zin) - mu c1
+    float sig, i1, i3x2;      // sigT, 1/sigT, 2/sigT^3
+    float c0s, c1s, c2s;      // source fit / sigT: q0 / sigT = c0s + zin (c1s + c2s zin)
+    float na1, m2s;           // -mu c1, 2 mu sigT:
+                              // mu q1 = mu (c1 + 2 c2 zin) = 2 mu sigT (c1s + c2s zin) - mu c1
     float q2m;                // mu2 q2: the integral's q2 term is
                               // (2 mu2 q2 / sigT^4) poly / 6 = mu2 q2 (2 ds^4) r4
 };
@@ -443,36 +444,41 @@ __device__ __forceinline__ RunK run_consts(const float4 t, int kind)
 {
 #pragma clang fp contract(off)
     RunK k;
+    float c0, c1, c2;
     k.sig = t.x;
     k.i1 = rcp(t.x);
-    k.i2x2 = 2.f * (k.i1 * k.i1);
-    fit_coeffs<STORE>(t, kind, k.c0, k.c1, k.c2);
-    k.na1 = -SMOC_MU * k.c1;
-    k.q2m = SMOC_MU2 * k.c2;
+    k.i3x2 = (2.f * (k.i1 * k.i1)) * k.i1;
+    fit_coeffs<STORE>(t, kind, c0, c1, c2);
+    k.c0s = c0 * k.i1;
+    k.c1s = c1 * k.i1;
+    k.c2s = c2 * k.i1;
+    k.na1 = -SMOC_MU * c1;
+    k.m2s = (2.f * SMOC_MU) * k.sig;
+    k.q2m = SMOC_MU2 * c2;
     return k;
 }
 
 // attenuate() with the run constants hoisted: same formulas, the products
 // of constants regrouped (rounding-level differences only).
 template <int EXPM>
-__device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zin, float zin2,
+__device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zin,
                                                float &psi, float &one_m_ev, const ExpTab &tb)
 {
 #pragma clang fp contract(off)
     const float tau = k.sig * ds;
     const EvTerms x = ev_terms<EXPM>(tau, tb);
-    const float e1 = x.ev * k.i1;
-    const float i2 = 0.5f * k.i2x2;  // 1/sigT^2 (this path is rare on the benchmark streams)
+    const float i2 = k.i1 * k.i1;  // 1/sigT^2 (this path is rare on the benchmark streams)
     const float m2 = x.tme * i2;
-    const float reuse = __builtin_fmaf(e1, k.i2x2, tau * (tau - 2.f));
-    const float q0 = __builtin_fmaf(k.c2, zin2, __builtin_fmaf(k.c1, zin, k.c0));
-    const float q1m = SMOC_MU * __builtin_fmaf(2.f * k.c2, zin, k.c1);
+    const float reuse = __builtin_fmaf(x.ev, k.i3x2, tau * (tau - 2.f));
+    const float inner = __builtin_fmaf(k.c2s, zin, k.c1s);
+    const float q0s = __builtin_fmaf(inner, zin, k.c0s);  // q0 / sigT
+    const float q1m = __builtin_fmaf(inner, k.m2s, k.na1);
     const float kfi6 = (2.f * k.q2m) * (i2 * i2);  // 2 mu2 q2 / sigT^4
-    const float flux_integral =
-        __builtin_fmaf(q0, m2, __builtin_fmaf(psi, e1, __builtin_fmaf(q1m, reuse,
-                                                          kfi6 * (x.poly * (1.f / 6.f)))));
-    psi = __builtin_fmaf(q0, e1, __builtin_fmaf(q1m, m2, __builtin_fmaf(k.q2m, reuse,
-                                                                        psi * x.one_m_ev)));
+    const float flux_integral = __builtin_fmaf(
+        q0s * k.sig, m2,
+        __builtin_fmaf(psi, x.ev * k.i1, __builtin_fmaf(q1m, reuse, kfi6 * (x.poly * (1.f / 6.f)))));
+    psi = __builtin_fmaf(q0s, x.ev, __builtin_fmaf(q1m, m2, __builtin_fmaf(k.q2m, reuse,
+                                                                          psi * x.one_m_ev)));
     one_m_ev = x.one_m_ev;
     return flux_integral;
 }
@@ -492,13 +498,16 @@ struct SegA {
 
 // ds2 = ds^2 and ds4x2 = 2 ds^4 are the segment's (shared by every group):
 // tau - ev = tau^2 r, so (tau - ev) / sigT^2 = ds^2 r, and the q2 term of
-// the integral, (2 mu2 q2 / sigT^4) tau^4 r4, is mu2 q2 (2 ds^4) r4.  The
-// psi-free part of the flux integral is added into the block sum bsum at the
-// end of its chain of fused multiply-adds, so it costs no add of its own and
-// no register per segment.
+// the integral, (2 mu2 q2 / sigT^4) tau^4 r4, is mu2 q2 (2 ds^4) r4.
+// With q0 / sigT from the fit (c0s, c1s, c2s), the source's q0 term is
+// (q0 / sigT) ev and needs no ev / sigT; reuse takes ev times 2 / sigT^3.
+// The psi-free part of the flux integral goes into three block sums, each a
+// single fused multiply-add per segment, whose run-constant factors are applied
+// once per block: b0 = sum (q0 / sigT) m2 (times sigT), bsum = sum mu q1 reuse
+// (times 1) and s2 = sum (2 ds^4) r4 (times mu2 q2).
 template <bool HAS_C2>
 __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, float ds4x2, float zin,
-                                         float &bsum)
+                                         float &b0, float &bsum, float &s2)
 {
 #pragma clang fp contract(off)
     const float tau = k.sig * ds;
@@ -507,21 +516,22 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
     const float r = __builtin_fmaf(t2, r4, __builtin_fmaf(tau, -1.f / 6.f, 0.5f));
     SegA a;
     a.ev = __builtin_fmaf(-t2, r, tau);  // one fused rounding
-    const float e1 = a.ev * k.i1;  // ev / sigT
-    const float m2 = ds2 * r;      // (tau - ev) / sigT^2
-    const float reuse = __builtin_fmaf(e1, k.i2x2, __builtin_fmaf(-2.f, tau, t2));
+    const float m2 = ds2 * r;  // (tau - ev) / sigT^2
+    const float reuse = __builtin_fmaf(a.ev, k.i3x2, __builtin_fmaf(-2.f, tau, t2));
     if constexpr (HAS_C2) {
         // Horner in zin: no zin^2 per segment; mu q1 from the same inner term
-        const float inner = __builtin_fmaf(k.c2, zin, k.c1);
-        const float q0 = __builtin_fmaf(inner, zin, k.c0);
-        const float q1m = __builtin_fmaf(inner, 2.f * SMOC_MU, k.na1);
-        bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(q1m, reuse,
-                                                     __builtin_fmaf(k.q2m * ds4x2, r4, bsum)));
-        a.src = __builtin_fmaf(q0, e1, __builtin_fmaf(q1m, m2, k.q2m * reuse));
+        const float inner = __builtin_fmaf(k.c2s, zin, k.c1s);
+        const float q0s = __builtin_fmaf(inner, zin, k.c0s);  // q0 / sigT
+        const float q1m = __builtin_fmaf(inner, k.m2s, k.na1);
+        b0 = __builtin_fmaf(q0s, m2, b0);
+        bsum = __builtin_fmaf(q1m, reuse, bsum);
+        s2 = __builtin_fmaf(ds4x2, r4, s2);
+        a.src = __builtin_fmaf(q0s, a.ev, __builtin_fmaf(q1m, m2, k.q2m * reuse));
     } else {
-        const float q0 = __builtin_fmaf(k.c1, zin, k.c0);
-        bsum = __builtin_fmaf(q0, m2, __builtin_fmaf(-k.na1, reuse, bsum));
-        a.src = __builtin_fmaf(q0, e1, -k.na1 * m2);
+        const float q0s = __builtin_fmaf(k.c1s, zin, k.c0s);
+        b0 = __builtin_fmaf(q0s, m2, b0);
+        bsum = __builtin_fmaf(-k.na1, reuse, bsum);
+        a.src = __builtin_fmaf(q0s, a.ev, -k.na1 * m2);
     }
     return a;
 }
@@ -757,10 +767,12 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
-                                float bs[GPL], ps[GPL];  // psi-free part; sum of psi ev
+                                // block sums: series_a's three psi-free parts and
+                                // series_b's sum of psi ev
+                                float b0[GPL], bs[GPL], s2[GPL], ps[GPL];
 #pragma unroll
                                 for (int j = 0; j < GPL; j++)
-                                    bs[j] = ps[j] = 0.f;
+                                    b0[j] = bs[j] = s2[j] = ps[j] = 0.f;
 #pragma unroll
                                 for (int u = 0; u < N; u++) {
                                     const RawSeg &sg = u == 0 ? cur : nx[u - 1];
@@ -769,7 +781,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
 #pragma unroll
                                     for (int j = 0; j < GPL; j++)
                                         a[u][j] = series_a<HAS_C2>(rk[j], sg.len, d2, d4x2, sg.zin,
-                                                                   bs[j]);
+                                                                   b0[j], bs[j], s2[j]);
                                 }
 #pragma unroll
                                 for (int j = 0; j < GPL; j++) {
@@ -780,10 +792,15 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                         series_b(a[u][j], psi[j], ps[j]);
                                     }
                                     // the block's flux integrals, summed in FP32, join
-                                    // the run's accumulator once
-                                    if constexpr (MODE != MODE_XFER)
-                                        acc_add_block(acc[j],
-                                                      __builtin_fmaf(ps[j], rk[j].i1, bs[j]), N);
+                                    // the run's accumulator once:
+                                    // bsum + mu2 q2 s2 + sigT b0 + (sum psi ev) / sigT
+                                    if constexpr (MODE != MODE_XFER) {
+                                        float t = bs[j];
+                                        if constexpr (HAS_C2)
+                                            t = __builtin_fmaf(s2[j], rk[j].q2m, t);
+                                        t = __builtin_fmaf(b0[j], rk[j].sig, t);
+                                        acc_add_block(acc[j], __builtin_fmaf(ps[j], rk[j].i1, t), N);
+                                    }
                                 }
                                 k += N;
                                 cur = nx[N - 1];
@@ -798,12 +815,11 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             else
                                 block(std::integral_constant<int, 1>{});
                         } else {
-                            const float zin2g = cur.zin * cur.zin;
 #pragma unroll
                             for (int j = 0; j < GPL; j++) {
                                 float m;
                                 const float fi = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
-                                                                     zin2g, psi[j], m, tb);
+                                                                     psi[j], m, tb);
                                 if constexpr (MODE == MODE_XFER)
                                     mprod[j] *= m;
                                 else
