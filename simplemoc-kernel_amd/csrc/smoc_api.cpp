@@ -1186,56 +1186,58 @@ static int run_once(smoc_ctx *c, const smoc_launch &l, smoc_timing *tm)
 
 // Measures candidate launch shapes on the loaded problem (each run is a
 // complete, correct sweep) and keeps the fastest: first the track split and
-// persistence, then groups per lane, then waves per workgroup.
+// persistence, then groups per lane, then waves per workgroup.  Candidates run
+// eagerly (the graph of the chosen shape replays the same kernels).  Each one
+// is timed in alternation with the current best, so that a drift of the GPU's
+// clocks during the search favours neither, and must beat it by more than the
+// timing noise.
 static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
 {
-    auto measure = [&](const smoc_launch &l, double *ms) {
+    auto time_once = [&](const smoc_launch &l, double *ms) {
         smoc_timing t{};
-        int rc = run_once(c, l, &t);  // warm-up (plans, graphs, caches)
-        if (rc != SMOC_OK)
-            return rc;
-        double best = 1e30;
-        for (int r = 0; r < 3; r++) {
-            rc = run_once(c, l, &t);
-            if (rc != SMOC_OK)
-                return rc;
-            best = std::min(best, t.total_ms);
-        }
-        *ms = best;
-        return SMOC_OK;
+        int rc = run_once(c, l, &t);
+        *ms = t.total_ms;
+        return rc;
     };
     smoc_launch best = base;
     best.autotune = 0;
+    best.use_graph = 0;
     best.chunk = 0;
     best.persistent = 0;
-    double best_ms = 0.0;
-    // a few runs first, so that the baseline is not timed on a GPU still
-    // raising its clocks (a candidate measured later would win on that alone)
-    for (int r = 0; r < 3; r++) {
-        smoc_timing t{};
-        int rc = run_once(c, best, &t);
-        if (rc != SMOC_OK)
-            return rc;
+    // runs of the starting shape first, for at least 200 ms of GPU time, so
+    // that no candidate is timed on a GPU still raising its clocks
+    {
+        double spent = 0.0, ms = 0.0;
+        for (int r = 0; r < 1000 && (r < 3 || spent < 200.0); r++) {
+            int rc = time_once(best, &ms);
+            if (rc != SMOC_OK)
+                return rc;
+            spent += ms;
+        }
     }
-    int rc = measure(best, &best_ms);
-    if (rc != SMOC_OK)
-        return rc;
     auto consider = [&](smoc_launch cand) {
-        double ms = 0.0;
         fill_launch(c, &cand, &cand);
         if (same_launch(cand, best))
             return SMOC_OK;
-        int r = measure(cand, &ms);
-        if (r == SMOC_ERR_ARG)  // combination not supported for this problem
+        double ms = 0.0, t_best = 1e30, t_cand = 1e30;
+        int r = time_once(cand, &ms);  // warm-up (plans, caches)
+        if (r == SMOC_ERR_ARG)         // combination not supported for this problem
             return SMOC_OK;
         if (r != SMOC_OK)
             return r;
-        if (ms < 0.99 * best_ms) {  // a change must gain more than timing noise
-            best_ms = ms;
-            best = cand;
+        for (int k = 0; k < 5; k++) {
+            if ((r = time_once(best, &ms)) != SMOC_OK)
+                return r;
+            t_best = std::min(t_best, ms);
+            if ((r = time_once(cand, &ms)) != SMOC_OK)
+                return r;
+            t_cand = std::min(t_cand, ms);
         }
+        if (t_cand < 0.99 * t_best)  // a change must gain more than timing noise
+            best = cand;
         return SMOC_OK;
     };
+    int rc = SMOC_OK;
     const int64_t mean_len = c->T > 0 ? c->S / c->T : 0;
     {
         smoc_launch cand = best;

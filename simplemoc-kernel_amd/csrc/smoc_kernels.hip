@@ -513,7 +513,10 @@ __device__ __forceinline__ SegA series_a(const RunK &k, float ds, float ds2, flo
     const float tau = k.sig * ds;
     const float r4 = r4_series(tau);
     const float t2 = tau * tau;
-    const float r = __builtin_fmaf(t2, r4, __builtin_fmaf(tau, -1.f / 6.f, 0.5f));
+    // r = 1/2 - tau/6 + tau^2 r4 in Horner form: the -1/6 is the literal of a
+    // two-operand FMA and 1/2 an inline constant, so no constant is held in an
+    // SGPR (VALU instructions that read SGPRs pair less often)
+    const float r = __builtin_fmaf(tau, __builtin_fmaf(tau, r4, -1.f / 6.f), 0.5f);
     SegA a;
     a.ev = __builtin_fmaf(-t2, r, tau);  // one fused rounding
     const float m2 = ds2 * r;  // (tau - ev) / sigT^2
