@@ -44,8 +44,11 @@ says so, timed over the same number of steps):
   * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
     the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
-    on this host's cores on a bounded sample of the same stream, plus the
-    accuracy of the GPU paths against the FP64 oracle (the checker).
+    on this host's cores on a bounded sample of the same stream (it fixes each
+    segment's length and axial position at its constants), with
+    `same_stream_port`: the C restatement on the same tracks with the stream's
+    own geometry; plus the accuracy of the GPU paths against the FP64 oracle
+    (the checker).
 """
 import argparse
 import json
@@ -172,12 +175,12 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
         ref.init(prob.G, prob.R, prob.F)
         ref.set_regions(prob.sigT, prob.fine_source)
 
-    def timed(segments, nthreads):
+    def timed(segments, nthreads, port=False):
         T = int(np.searchsorted(prob.track_off, segments, side="right")) - 1
         T = max(1, min(T, prob.T))
         S = int(prob.track_off[T])
         F = prob.F
-        if use_ref:
+        if use_ref and not port:
             zf = (prob.z[:S] * np.float32(F)).astype(np.float32)
             fai = np.clip(zf.astype(np.int32), 0, F - 1).astype(np.int32)
             sec, _ = ref.sweep(prob.track_off[: T + 1], prob.region[:S], fai, prob.psi_in[:T],
@@ -206,6 +209,15 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
             "single_thread": {"value": S1 * prob.G / sec1, "cores": 1,
                               "sample": f"first {T1} tracks ({S1} segments), same code, "
                                         f"{sec1:.3f} s"}}
+    if use_ref:
+        # the same tracks through the C restatement, which takes each segment's
+        # own length and axial position (the reference fixes them)
+        Tp, Sp, secp = timed(sample_segments, threads, port=True)
+        base["same_stream_port"] = {
+            "value": Sp * prob.G / secp, "cores": threads, "kind": "port",
+            "sample": f"first {Tp} tracks ({Sp} segments), the C restatement of "
+                      f"attenuate_segment (oracle_baseline_f32, OpenMP over tracks) with the "
+                      f"stream's own lengths and axial positions; {secp:.3f} s"}
     acc = None
     if accuracy:
         acc = {}
