@@ -131,7 +131,8 @@ def measured_traffic():
     """HBM bytes per k_sweep launch from the committed rocprofv3 summary:
     FETCH_SIZE x its copy-kernel calibration factor + WRITE_SIZE x its."""
     try:
-        txt = open(TRAFFIC_SUMMARY).read()
+        with open(TRAFFIC_SUMMARY) as fh:
+            txt = fh.read()
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", txt)
@@ -142,7 +143,8 @@ def mixed_valu_instructions():
     """VALU instructions per k_sweep launch of the mixed workload from the
     committed rocprofv3 summary (SQ_INSTS_VALU; fixed for a given input)."""
     try:
-        txt = open(MIXED_SUMMARY).read()
+        with open(MIXED_SUMMARY) as fh:
+            txt = fh.read()
     except OSError:
         return None
     m = re.search(r"k_sweep\s+SQ_INSTS_VALU\s+([0-9.eE+]+)", txt)
