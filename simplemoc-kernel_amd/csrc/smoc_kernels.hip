@@ -1314,31 +1314,59 @@ __global__ void __launch_bounds__(1024) k_reduce(smoc_reduce_args a)
     const int Gp = a.Gp;
     const int64_t rb = a.fsr_run_off[i], re = a.fsr_run_off[i + 1];
     const int64_t *__restrict__ idx = a.fsr_run_idx;
+    if constexpr (TALLY == SMOC_TALLY_DET_F64) {
+        // both group slots in one loop, 8 runs of loads in flight per slot
+        // before their adds; each slot's sum still takes the runs in
+        // ascending order.  Padding lanes load group G-1's entries (the same
+        // cache lines as the last real group) and store nothing.
+        using part_t = typename std::conditional<SINGLE, float, double>::type;
+        const part_t *__restrict__ part = static_cast<const part_t *>(a.part);
+        int gi[GPL];
+        bool on[GPL];
+        double s[GPL];
+#pragma unroll
+        for (int j = 0; j < GPL; j++) {
+            const int g = sl * 64 * GPL + j * 64 + lane;
+            on[j] = g < a.G;
+            gi[j] = on[j] ? g : a.G - 1;
+            s[j] = a.init && on[j] ? (double)a.init[i * Gp + g] : 0.0;
+        }
+        int64_t r = rb;
+        for (; r + 8 <= re; r += 8) {
+            part_t v[8][GPL];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const part_t *row = part + idx[r + u] * Gp;
+#pragma unroll
+                for (int j = 0; j < GPL; j++)
+                    v[u][j] = row[gi[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < GPL; j++)
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    s[j] += (double)v[u][j];
+        }
+        for (; r < re; ++r) {
+            const part_t *row = part + idx[r] * Gp;
+#pragma unroll
+            for (int j = 0; j < GPL; j++)
+                s[j] += (double)row[gi[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < GPL; j++)
+            if (on[j])
+                static_cast<double *>(a.flux)[i * Gp + sl * 64 * GPL + j * 64 + lane] = s[j];
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         const int g = sl * 64 * GPL + j * 64 + lane;
         if (g >= a.G)
             continue;
         const int64_t o = i * Gp + g;
-        if constexpr (TALLY == SMOC_TALLY_DET_F64) {
-            using part_t = typename std::conditional<SINGLE, float, double>::type;
-            const part_t *part = static_cast<const part_t *>(a.part);
-            double s = a.init ? (double)a.init[o] : 0.0;
-            int64_t r = rb;
-            for (; r + 4 <= re; r += 4) {
-                const double v0 = part[idx[r] * Gp + g];
-                const double v1 = part[idx[r + 1] * Gp + g];
-                const double v2 = part[idx[r + 2] * Gp + g];
-                const double v3 = part[idx[r + 3] * Gp + g];
-                s += v0;
-                s += v1;
-                s += v2;
-                s += v3;
-            }
-            for (; r < re; ++r)
-                s += (double)part[idx[r] * Gp + g];
-            static_cast<double *>(a.flux)[o] = s;
-        } else {
+        {
+            // DET_KAHAN: compensated FP32, one slot at a time
 #pragma clang fp contract(off)
             float s = a.init ? a.init[o] : 0.f, c = 0.f;
             for (int64_t r = rb; r < re; ++r) {
