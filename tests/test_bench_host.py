@@ -2,7 +2,10 @@
 the committed profile summaries it reads parse, and the CPU baseline leg
 returns well-formed figures on a small sample."""
 import importlib.util
+import json
 import os
+import subprocess
+import sys
 
 import pytest
 
@@ -40,10 +43,24 @@ def test_committed_profiles_parse():
 
 
 def test_cpu_leg_small_sample():
-    b = _bench()
-    prob = smoc.generate(60_000, G=32, R=200, F=5, seed=3)
-    base, acc = b.cpu_leg(prob, 40_000, 10_000, None, None, False)
-    assert acc is None
+    # in a child process: the reference build is compiled with -Ofast, and loading
+    # it switches the process to flush-to-zero arithmetic
+    code = (
+        "import importlib.util, json, os, sys\n"
+        f"sys.path[:0] = [{os.path.join(ROOT, 'simplemoc-kernel_amd')!r}, "
+        f"{os.path.join(ROOT, 'oracle')!r}, {os.path.join(ROOT, 'tests')!r}]\n"
+        "import smoc\n"
+        f"spec = importlib.util.spec_from_file_location('b', {os.path.join(ROOT, 'bench.py')!r})\n"
+        "b = importlib.util.module_from_spec(spec); spec.loader.exec_module(b)\n"
+        "prob = smoc.generate(60_000, G=32, R=200, F=5, seed=3)\n"
+        "base, acc = b.cpu_leg(prob, 40_000, 10_000, None, None, False)\n"
+        "print(json.dumps({'base': base, 'acc': acc}))\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    base = res["base"]
+    assert res["acc"] is None
     assert base["value"] > 0 and base["cores"] >= 1
     assert base["unit"] == "segment-group attenuations/s"
     assert base["single_thread"]["value"] > 0 and base["single_thread"]["cores"] == 1
