@@ -16,6 +16,7 @@
 
 #include "smoc.h"
 #include "smoc_internal.h"
+#include "smoc_plan.h"
 
 namespace {
 
@@ -130,6 +131,11 @@ struct smoc_ctx {
     // deterministic tally bookkeeping
     int64_t n_runs = 0;
     DevBuf<int64_t> track_run_off, fsr_run_off, fsr_run_idx;
+    // region-sorted stream: the run id at each track's split points (every
+    // SMOC_RUN_SPLIT segments from its start), track by track; host copies of
+    // it and of track_run_off for the chunk plan
+    DevBuf<int64_t> split_run;
+    std::vector<int64_t> h_track_run_off, h_split_run;
     DevBuf<unsigned char> part;
 
     // outer (source) iteration
@@ -460,6 +466,7 @@ int smoc_destroy(smoc_ctx *c)
     c->track_run_off.release();
     c->fsr_run_off.release();
     c->fsr_run_idx.release();
+    c->split_run.release();
     c->part.release();
     c->queue.release();
     c->plan.release();
@@ -605,36 +612,54 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
 }
 
 // Run bookkeeping of the deterministic tally: runs are numbered track by
-// track in stream order; a run is a maximal stretch of one track's segments
-// with the same (region, interval) when c->runs, else a single segment.
-// fsr_run_idx lists, for every (region, interval), its runs in ascending id.
+// track in stream order; when c->runs a run is a maximal stretch of one
+// track's segments with the same (region, interval) that does not cross a
+// split point (a stream index that is a multiple of SMOC_RUN_SPLIT: the sweep
+// ends its runs there too), else a single segment.  fsr_run_idx lists, for
+// every (region, interval), its runs in ascending id; split_run the id of the
+// run that begins at each split point inside a track, track by track.
+static_assert((SMOC_RUN_SPLIT & (SMOC_RUN_SPLIT - 1)) == 0, "split length: a power of two");
 static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float *z, int64_t T,
                       const int64_t *track_off)
 {
     const int F = c->p.fine_axial;
     const int64_t K = c->n_fsr();
     std::vector<int32_t> run_key;
-    std::vector<int64_t> tro((size_t)T + 1);
+    std::vector<int64_t> tro((size_t)T + 1), split;
     try {
         run_key.reserve(c->runs ? (size_t)T + 1024 : (size_t)S);
+        if (c->runs)
+            split.reserve((size_t)(S / SMOC_RUN_SPLIT) + 1);
+        for (int64_t t = 0; t < T; t++) {
+            tro[t] = (int64_t)run_key.size();
+            int32_t prev = -1;
+            for (int64_t k = track_off[t]; k < track_off[t + 1]; k++) {
+                int f;
+                float zin;
+                smoc_decode_z(z[k], F, &f, &zin);
+                const int32_t key = region[k] * F + f;
+                const bool at_split =
+                    c->runs && k > track_off[t] && (k & (SMOC_RUN_SPLIT - 1)) == 0;
+                if (at_split)
+                    split.push_back((int64_t)run_key.size());
+                if (!c->runs || key != prev || at_split)
+                    run_key.push_back(key);
+                prev = key;
+            }
+        }
+        c->h_track_run_off.clear();
+        c->h_split_run.clear();
+        if (c->runs) {
+            c->h_track_run_off = tro;
+            c->h_split_run = split;
+        }
     } catch (...) {
         smoc_set_error("smoc_set_segments: host allocation for run bookkeeping failed");
         return SMOC_ERR_NOMEM;
     }
-    for (int64_t t = 0; t < T; t++) {
-        tro[t] = (int64_t)run_key.size();
-        int32_t prev = -1;
-        for (int64_t k = track_off[t]; k < track_off[t + 1]; k++) {
-            int f;
-            float zin;
-            smoc_decode_z(z[k], F, &f, &zin);
-            const int32_t key = region[k] * F + f;
-            if (!c->runs || key != prev)
-                run_key.push_back(key);
-            prev = key;
-        }
-    }
     tro[T] = (int64_t)run_key.size();
+    if (c->runs)
+        c->h_track_run_off[T] = tro[T];
     const int64_t n_runs = tro[T];
     std::vector<int64_t> fro((size_t)K + 1, 0), idx((size_t)std::max<int64_t>(n_runs, 1));
     for (int64_t r = 0; r < n_runs; r++)
@@ -652,9 +677,14 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     if (rc == SMOC_OK)
         rc = c->fsr_run_idx.alloc((size_t)std::max<int64_t>(n_runs, 1));
     if (rc == SMOC_OK)
+        rc = c->split_run.alloc(std::max<size_t>(split.size(), 1));
+    if (rc == SMOC_OK)
         rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
     if (rc != SMOC_OK)
         return rc;
+    if (!split.empty())
+        HIPCHK(hipMemcpy(c->split_run.p, split.data(), sizeof(int64_t) * split.size(),
+                         hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->track_run_off.p, tro.data(), sizeof(int64_t) * (T + 1),
                      hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->fsr_run_off.p, fro.data(), sizeof(int64_t) * (K + 1),
@@ -741,6 +771,8 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
     c->n_runs = 0;
     c->max_track = 0;
     c->h_track_off.clear();
+    c->h_track_run_off.clear();
+    c->h_split_run.clear();
     // the stream arrays carry kStreamPad zeroed entries past S: the sweep
     // loads one segment ahead without a bounds branch
     int rc = c->len.alloc((size_t)S + kStreamPad);
@@ -834,11 +866,15 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
     *l = d;
 }
 
-// Work items of a chunked sweep: every track becomes ceil(len / C) chunks
-// (an empty track one empty chunk).  Phase 1 sweeps each track's first chunk
-// (a multi-chunk track's first chunk also emits its psi_out as a transfer)
-// and runs the transfer pass of the middle chunks; phase 2 sweeps the
-// remaining chunks.  Both lists are ordered longest first.
+// Work items of a chunked sweep: every track is cut into chunks of about C
+// segments (smoc_plan.h; an empty track is one empty chunk).  Phase 1 sweeps
+// each track's first chunk (a multi-chunk track's first chunk also emits its
+// psi_out as a transfer) and runs the transfer pass of the middle chunks;
+// phase 2 sweeps the remaining chunks.  Both lists are ordered longest first.
+// An item's first run id (used by the deterministic tally) is its first
+// segment, where every segment is a run, or on a region-sorted stream the run
+// that begins at its first segment: C is then a multiple of SMOC_RUN_SPLIT, so
+// each chunk after a track's first begins at one of the track's split points.
 static int build_plan(smoc_ctx *c, int C)
 {
     Plan &pl = c->plan;
@@ -852,17 +888,25 @@ static int build_plan(smoc_ctx *c, int C)
     std::vector<int64_t> items, tio((size_t)c->T + 1);
     std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
     const std::vector<int64_t> &off = c->h_track_off;
-    int64_t n = 0;
+    const bool split_ids = c->det() && c->runs;
+    if (split_ids && C % SMOC_RUN_SPLIT != 0) {  // refused by check_launch
+        smoc_set_error("smoc_run: chunk %d is not a multiple of %d", C, SMOC_RUN_SPLIT);
+        return SMOC_ERR_ARG;
+    }
+    int64_t n = 0, sp = 0;
     for (int64_t t = 0; t < c->T; t++) {
         tio[t] = n;
         const int64_t b = off[t], e = off[t + 1];
-        const int64_t nc = std::max<int64_t>(1, (e - b + C - 1) / C);
+        const int64_t nc = smoc_chunk_count(b, e, C);
         for (int64_t k = 0; k < nc; k++, n++) {
-            const int64_t cb = b + k * C, ce = std::min<int64_t>(e, cb + C);
+            const int64_t cb = smoc_chunk_begin(b, k, C);
+            const int64_t ce = smoc_chunk_end(b, e, k, C);
             items.push_back(t);
             items.push_back(cb);
             items.push_back(ce);
-            items.push_back(cb);  // first run id: runs are segments (checked by caller)
+            items.push_back(!split_ids ? cb
+                            : k == 0   ? c->h_track_run_off[t]
+                                       : c->h_split_run[sp + smoc_split_index(b, cb) - 1]);
             if (k == 0)
                 p1.emplace_back(ce - cb, (n << 2) | (nc > 1 ? 2 : 0));
             else
@@ -870,6 +914,7 @@ static int build_plan(smoc_ctx *c, int C)
             if (k > 0 && k + 1 < nc)
                 p1.emplace_back(ce - cb, (n << 2) | 1);
         }
+        sp += smoc_split_count(b, e);
     }
     tio[c->T] = n;
     auto longest_first = [](const std::pair<int64_t, int64_t> &x,
@@ -929,12 +974,15 @@ struct RunBufs {
 static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *what)
 {
     const bool runs = c->runs && !l.segment_flush;
-    // FIXED64 too: a chunk boundary would split a run's sum, and the run sums
-    // are what is rounded to fixed point, so results would depend on the chunk
-    if (l.chunk > 0 && (c->det() || c->fixed()) && runs) {
-        smoc_set_error("%s: chunked sweeps of a region-sorted stream need an atomic "
-                       "FP32/FP64 tally (deterministic runs would cross chunk boundaries)",
-                       what);
+    // FIXED64 too: the run sums are what is rounded to fixed point.  Runs end
+    // at every SMOC_RUN_SPLIT-th segment of a track, so chunks of a multiple
+    // of that length split no run, and others would make results depend on
+    // the chunk length.
+    if (l.chunk > 0 && (c->det() || c->fixed()) && runs && l.chunk % SMOC_RUN_SPLIT != 0) {
+        smoc_set_error("%s: chunked sweeps of a region-sorted stream with a deterministic "
+                       "or FIXED64 tally need a chunk length that is a multiple of %d "
+                       "(got %d)",
+                       what, SMOC_RUN_SPLIT, l.chunk);
         return SMOC_ERR_ARG;
     }
     if (l.segment_flush && c->det() && c->runs) {
@@ -1038,8 +1086,10 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
         HIPCHK(hipEventRecord(ev[1], c->stream));
     if (chunked) {
         if (b.plan_chunk > 0)
-            LAUNCHCHK(smoc_launch_plan(c->track_off.p, c->T, b.plan_chunk, b.items, b.list1,
-                                       b.list2, b.track_item_off, c->stream));
+            LAUNCHCHK(smoc_launch_plan(
+                c->track_off.p, c->T, b.plan_chunk, b.items, b.list1, b.list2,
+                b.track_item_off, c->det() && c->runs ? c->track_run_off.p : nullptr,
+                c->det() && c->runs ? c->split_run.p : nullptr, c->stream));
         // phase 1: first chunks (swept from psi_in) + transfers of middle chunks
         a.items = b.items;
         a.xfer = b.xfer;
@@ -1383,8 +1433,7 @@ int smoc_run_noalloc(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing
         const int64_t C = l.chunk;
         int64_t n = 0, n1 = 0, n2 = 0;
         for (int64_t t = 0; t < c->T; t++) {
-            const int64_t len = c->h_track_off[t + 1] - c->h_track_off[t];
-            const int64_t nc = len > C ? (len + C - 1) / C : 1;
+            const int64_t nc = smoc_chunk_count(c->h_track_off[t], c->h_track_off[t + 1], C);
             n += nc;
             n1 += nc > 1 ? nc - 1 : 1;
             n2 += nc - 1;

@@ -21,6 +21,15 @@ extern "C" {
 /* exp table capacity in floats (the reference table has N = 353 intervals) */
 #define SMOC_TABLE_CAP 1024
 
+/* On a region-sorted stream a run (the stretch of one track in one region
+ * interval whose tally is summed in registers) also ends before every stream
+ * index that is a multiple of SMOC_RUN_SPLIT (a split point).  For a chunk
+ * length C that is a multiple of SMOC_RUN_SPLIT, chunked sweeps cut tracks at
+ * split points only (smoc_plan.h), so every chunk begins a run, and the run
+ * partition, with it every deterministic and FIXED64 sum, is the same whether
+ * the tracks are chunked or not.  A power of two. */
+#define SMOC_RUN_SPLIT 1024
+
 void smoc_set_error(const char *fmt, ...);
 
 /* Arguments of the attenuation sweep.  One work item is (track or chunk,
@@ -118,9 +127,13 @@ int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *str
 int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F, int64_t S, void *stream);
 /* chunk plan of a chunked sweep on the device (k_plan): items [n][4], the
  * phase-1 / phase-2 work lists and track_item_off [T+1], for chunk length
- * chunk >= 1; the sizes follow from track_off as for the host plan */
+ * chunk >= 1; the sizes follow from track_off as for the host plan.  Each
+ * item's first run id is its first segment, or, with track_run_off and
+ * split_run given (deterministic tally of a region-sorted stream; chunk a
+ * multiple of SMOC_RUN_SPLIT), the id of the run that begins there */
 int smoc_launch_plan(const int64_t *track_off, int64_t T, int chunk, int64_t *items,
-                     int64_t *list1, int64_t *list2, int64_t *track_item_off, void *stream);
+                     int64_t *list1, int64_t *list2, int64_t *track_item_off,
+                     const int64_t *track_run_off, const int64_t *split_run, void *stream);
 int smoc_launch_fixed_to_f64(int64_t *flux, const float *init, int64_t n, void *stream);
 /* plain copy of n16 16-byte elements, kernel variant 0..SMOC_COPY_VARIANTS-1 */
 int smoc_launch_copy(const void *src, void *dst, int64_t n16, int variant, int cus,

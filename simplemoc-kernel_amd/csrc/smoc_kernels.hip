@@ -33,6 +33,7 @@
 
 #include "smoc.h"
 #include "smoc_internal.h"
+#include "smoc_plan.h"
 
 namespace {
 
@@ -690,7 +691,12 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         // device arrays are padded, so loads past the item's end are in
         // bounds; blocks never reach past it), and up to kSeries consecutive
         // segments that all take the series branch have their psi-free parts
-        // evaluated together before their psi updates run in order.
+        // evaluated together before their psi updates run in order.  A run
+        // also ends before every stream index that is a multiple of
+        // SMOC_RUN_SPLIT (the run table of the deterministic tally splits
+        // there too), so a chunk whose length is a multiple of that length,
+        // cut at split points, begins a run and the blocks, and the sums, do
+        // not depend on the chunking.
         if (kb32 < ke32) {
             RunK rk[GPL];
             uint32_t k = kb32;
@@ -766,7 +772,9 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                        series_ok(nx[n - 1]))
                                     ++n;
                             }
-                            n = min(n, ke32 - k);  // segments k .. k + n - 1 are the item's
+                            // segments k .. k + n - 1 are the item's and
+                            // cross no split point
+                            n = min(n, min(ke32 - k, SMOC_RUN_SPLIT - (k & (SMOC_RUN_SPLIT - 1))));
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
@@ -831,7 +839,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             ++k;
                             cur = nx[0];
                         }
-                        if (k >= ke32 || cur.code != ccode)
+                        if (k >= ke32 || (k & (SMOC_RUN_SPLIT - 1)) == 0 || cur.code != ccode)
                             break;
                     }
                 };
@@ -985,61 +993,71 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
 // ---- chunk plan built on the device ---------------------------------------------
 // The work items of a chunked sweep, written without host memory (the
 // allocation-free entry point builds them in its scratch on every call).  Track
-// t becomes nc = max(1, ceil(len / C)) chunks, items [i][4] = (track, begin,
-// end, first run id = begin); list1 holds each track's first chunk (ITEM_EMIT
+// t is cut into smoc_chunk_count chunks (smoc_plan.h), items [i][4] = (track,
+// begin, end, first run id), list1 holds each track's first chunk (ITEM_EMIT
 // when it has more) and its middle chunks (ITEM_XFER), list2 every chunk after
-// the first, tio[t] the first chunk of track t and tio[T] the chunk count.
-// These are the items of the host plan; the lists are in track order.  One
-// workgroup of 1024 threads: thread i takes a contiguous range of tracks,
-// counts its chunks and list entries, a scan over the threads gives its
+// the first, tio[t] the first chunk of track t and tio[T] the chunk count.  The
+// first run id is begin, or, given the run tables of a region-sorted stream (C
+// a multiple of SMOC_RUN_SPLIT, so every chunk after a track's first begins at
+// a split point), track_run_off[t] for the first chunk and the split_run entry
+// of the chunk's split point for the others; split_run holds, track by track,
+// the run id at each of the track's smoc_split_count split points.  These are
+// the items of the host plan; the lists are in track order.  One workgroup of
+// 1024 threads: thread i takes a contiguous range of tracks, counts its
+// chunks, list entries and split points, a scan over the threads gives its
 // offsets, then it writes.
 __global__ void __launch_bounds__(1024)
     k_plan(const int64_t *__restrict__ track_off, int64_t T, int C, int64_t *__restrict__ items,
-           int64_t *__restrict__ list1, int64_t *__restrict__ list2, int64_t *__restrict__ tio)
+           int64_t *__restrict__ list1, int64_t *__restrict__ list2, int64_t *__restrict__ tio,
+           const int64_t *__restrict__ track_run_off, const int64_t *__restrict__ split_run)
 {
-    __shared__ int64_t s_n[1024], s_1[1024], s_2[1024];
+    __shared__ int64_t s_n[1024], s_1[1024], s_2[1024], s_s[1024];
     const int tid = threadIdx.x;
     const int64_t per = (T + 1023) / 1024;
     const int64_t t0 = min(T, (int64_t)tid * per), t1 = min(T, t0 + per);
-    auto chunks = [&](int64_t t) {
-        const int64_t len = track_off[t + 1] - track_off[t];
-        return len > C ? (len + C - 1) / C : (int64_t)1;
-    };
-    int64_t n = 0, n1 = 0, n2 = 0;
+    auto chunks = [&](int64_t t) { return smoc_chunk_count(track_off[t], track_off[t + 1], C); };
+    auto splits = [&](int64_t t) { return smoc_split_count(track_off[t], track_off[t + 1]); };
+    int64_t n = 0, n1 = 0, n2 = 0, ns = 0;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t nc = chunks(t);
         n += nc;
         n1 += nc > 1 ? nc - 1 : 1;
         n2 += nc - 1;
+        ns += splits(t);
     }
     s_n[tid] = n;
     s_1[tid] = n1;
     s_2[tid] = n2;
+    s_s[tid] = ns;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan over the threads
-        int64_t a = 0, b = 0, c = 0;
+        int64_t a = 0, b = 0, c = 0, e = 0;
         if (tid >= d) {
             a = s_n[tid - d];
             b = s_1[tid - d];
             c = s_2[tid - d];
+            e = s_s[tid - d];
         }
         __syncthreads();
         s_n[tid] += a;
         s_1[tid] += b;
         s_2[tid] += c;
+        s_s[tid] += e;
         __syncthreads();
     }
-    int64_t i = s_n[tid] - n, p1 = s_1[tid] - n1, p2 = s_2[tid] - n2;
+    int64_t i = s_n[tid] - n, p1 = s_1[tid] - n1, p2 = s_2[tid] - n2, sp = s_s[tid] - ns;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t b = track_off[t], e = track_off[t + 1], nc = chunks(t);
         tio[t] = i;
         for (int64_t k = 0; k < nc; ++k) {
-            const int64_t cb = b + k * C, ce = cb + C < e ? cb + C : e;
+            const int64_t cb = smoc_chunk_begin(b, k, C), ce = smoc_chunk_end(b, e, k, C);
             int64_t *it = items + 4 * (i + k);
             it[0] = t;
             it[1] = cb;
             it[2] = ce;
-            it[3] = cb;  // first run id: every run is one segment (chunking is refused otherwise)
+            it[3] = !split_run ? cb
+                    : k == 0   ? track_run_off[t]
+                               : split_run[sp + smoc_split_index(b, cb) - 1];
         }
         list1[p1++] = (i << 2) | (nc > 1 ? ITEM_EMIT : 0);
         for (int64_t k = 1; k + 1 < nc; ++k)
@@ -1047,6 +1065,7 @@ __global__ void __launch_bounds__(1024)
         for (int64_t k = 1; k < nc; ++k)
             list2[p2++] = (i + k) << 2;
         i += nc;
+        sp += splits(t);
     }
     if (tid == 1023)  // the last thread's range ends at T: i is the chunk count
         tio[T] = i;
@@ -1580,12 +1599,16 @@ extern "C" int smoc_launch_encode_stream(float *zin_io, int32_t *code_io, int F,
 
 extern "C" int smoc_launch_plan(const int64_t *track_off, int64_t T, int chunk, int64_t *items,
                                 int64_t *list1, int64_t *list2, int64_t *track_item_off,
+                                const int64_t *track_run_off, const int64_t *split_run,
                                 void *stream)
 {
     if (T <= 0 || chunk <= 0)
         return T < 0 || chunk <= 0 ? (int)hipErrorInvalidValue : 0;
+    if (split_run && (!track_run_off || chunk % SMOC_RUN_SPLIT != 0))
+        return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
-                       track_off, T, chunk, items, list1, list2, track_item_off);
+                       track_off, T, chunk, items, list1, list2, track_item_off, track_run_off,
+                       split_run);
     return (int)hipGetLastError();
 }
 

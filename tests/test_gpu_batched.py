@@ -95,14 +95,79 @@ def test_autotune_picks_valid_launch(gpu_ready, ragged, ragged_oracle):
     assert normwise(flux, of) < 1e-5 and normwise(psi, op) < 1e-5
 
 
-def test_sorted_det_refuses_chunks(gpu_ready):
-    prob = smoc.generate(20_000, G=64, R=50, F=5, seed=2, sorted_stream=True, track_len=2000)
-    with smoc.Context(64, 50, 5, tally="det-f64") as ctx:
-        ctx.load(prob)
-        with pytest.raises(smoc.SmocError) as e:
-            ctx.run(chunk=100)
-        assert e.value.code == smoc.ERR_ARG
-        ctx.run(chunk=100 * 0)  # whole tracks work
+@pytest.fixture(scope="module")
+def long_runs():
+    # lognormal batches sorted by (region, interval) over few regions: runs of
+    # a few hundred segments, many of them crossing a split point (a stream
+    # index that is a multiple of 1024, where a run also ends)
+    return smoc.generate(0, G=100, R=40, F=5, seed=24, lognormal_tracks=48, median=600.0,
+                         sigma=1.0, lo=16, hi=8000, sorted_stream=True)
+
+
+@pytest.fixture(scope="module")
+def long_runs_oracle(long_runs):
+    return pyoracle.sweep_f64(long_runs, pyoracle.EXP_NATIVE, nthreads=8)
+
+
+@pytest.mark.parametrize("chunk", [0, 1024, 4096])
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("tally", ["det-f64", "det-kahan", "fixed64"])
+def test_sorted_chunked_vs_oracle(gpu_ready, long_runs, long_runs_oracle, chunk, persistent,
+                                  tally):
+    """Region-sorted stream, tallies that promise bit-exact results, tracks cut
+    at multiples of the chunk length (a multiple of 1024, so every chunk
+    begins a run): against the FP64 oracle."""
+    print(f"case sorted chunk={chunk} persistent={persistent} {tally} S={long_runs.S}",
+          flush=True)
+    flux, psi, tm = sweep(long_runs, tally, chunk=chunk, persistent=persistent)
+    of, op = long_runs_oracle
+    print(f"  normwise {normwise(flux, of):.2e} / {normwise(psi, op):.2e}", flush=True)
+    assert tm.used.chunk == chunk
+    assert normwise(flux, of) < 1e-5 and normwise(psi, op) < 1e-5
+
+
+@pytest.mark.parametrize("tally", ["det-f64", "fixed64"])
+def test_sorted_chunked_bit_exact_across_shapes(gpu_ready, long_runs, tally):
+    """For one chunk length every launch shape, the persistent kernel and a
+    graph replay give the same bits; chunked and whole-track sweeps differ at
+    rounding level only (chunks start from the psi of the transfer scan)."""
+    print(f"case sorted chunked shapes {tally}", flush=True)
+    with smoc.Context(long_runs.G, long_runs.R, long_runs.F, tally=tally) as ctx:
+        ctx.load(long_runs)
+        ref = None
+        for launch in ({"chunk": 2048},
+                       {"chunk": 2048, "persistent": 1, "waves_per_block": 4},
+                       {"chunk": 2048, "groups_per_lane": 1, "waves_per_block": 2,
+                        "reduce_waves": 8},
+                       {"chunk": 2048, "persistent": 1, "use_graph": 1},
+                       {"chunk": 2048, "persistent": 1, "use_graph": 1}):
+            tm = ctx.run(**launch)
+            assert tm.used.chunk == 2048
+            out = ctx.results()
+            if ref is None:
+                ref = out
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), launch
+        ctx.run()
+        whole = ctx.results()
+    assert normwise(ref[0], whole[0]) < 2e-6 and normwise(ref[1], whole[1]) < 2e-6
+
+
+def test_sorted_chunk_lengths_refused(gpu_ready, long_runs):
+    """On a region-sorted stream the deterministic and FIXED64 tallies take
+    chunk lengths that are multiples of 1024 only (others would cut runs, so
+    results would depend on the chunk length); atomic tallies take any."""
+    for tally in ("det-f64", "det-kahan", "fixed64"):
+        with smoc.Context(long_runs.G, long_runs.R, long_runs.F, tally=tally) as ctx:
+            ctx.load(long_runs)
+            for bad in (100, 1000, 3000):
+                with pytest.raises(smoc.SmocError) as e:
+                    ctx.run(chunk=bad)
+                assert e.value.code == smoc.ERR_ARG and "multiple of 1024" in str(e.value)
+            assert ctx.run(chunk=3072).used.chunk == 3072
+            ctx.run()  # whole tracks
+    with smoc.Context(long_runs.G, long_runs.R, long_runs.F, tally="atomic-f32") as ctx:
+        ctx.load(long_runs)
+        assert ctx.run(chunk=100).used.chunk == 100
 
 
 @pytest.mark.parametrize("gpl", [1, 2])

@@ -455,7 +455,8 @@ def test_sorted_block_boundaries(gpu_ready, seed):
                 ref = out
             assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]), \
                 (tally, gpl, wpb, rw)
-    # chunking would split runs: refused for the tallies that promise bit-exact results
+    # chunks of a length that is not a multiple of 1024 would cut runs: refused
+    # for the tallies that promise bit-exact results
     with smoc.Context(prob.G, prob.R, prob.F, tally="fixed64") as ctx:
         ctx.load(prob)
         with pytest.raises(smoc.SmocError) as e:

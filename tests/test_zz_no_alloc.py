@@ -56,6 +56,25 @@ def test_noalloc_matches_run_then_smaller_input(gpu_ready, tally):
             check_pair(ctx, tally, launch)
 
 
+@pytest.mark.parametrize("tally", ["det-f64", "fixed64"])
+def test_noalloc_sorted_chunked(gpu_ready, tally):
+    """Region-sorted stream in chunks of a multiple of 1024: the plan built on
+    the device takes each chunk's first run id from the run tables as the host
+    plan does; then a smaller input."""
+    sizes = ((40, 900.0, 9000), (13, 900.0, 4000))
+    probs = [smoc.generate(0, G=100, R=40, F=5, seed=44 + i, lognormal_tracks=n, median=m,
+                           sigma=1.0, lo=16, hi=h, sorted_stream=True)
+             for i, (n, m, h) in enumerate(sizes)]
+    with smoc.Context(100, 40, 5, tally=tally) as ctx:
+        ctx.set_regions(probs[0].sigT, probs[0].fine_source)
+        for i, prob in enumerate(probs):
+            ctx.set_segments(prob.len, prob.region, prob.z, prob.track_off, prob.psi_in)
+            for launch in ({"chunk": 1024, "persistent": 1}, {"chunk": 2048}):
+                print(f"case noalloc sorted {tally} input {i} {launch}", flush=True)
+                tm = check_pair(ctx, tally, launch)
+                assert tm.used.chunk == launch["chunk"]
+
+
 def test_noalloc_two_contexts_take_turns(gpu_ready):
     """Two contexts on one GPU share the array; calls one after another do
     not disturb each other (each rebuilds its plan in the array)."""
