@@ -74,8 +74,11 @@ extern "C" {
  *               segments join the FP64 run sum) and stored; a segmented
  *               reduction then sums the runs of every (region, interval) in
  *               a fixed order.  Bit-exact run to run and across launch
- *               shapes; chunked sweeps of a region-sorted stream are refused
- *               (SMOC_ERR_ARG), since chunks would split runs.
+ *               shapes.  On a region-sorted stream a run also ends before
+ *               every stream index that is a multiple of 1024, and chunked
+ *               sweeps need a chunk length that is a multiple of 1024
+ *               (others would split runs: SMOC_ERR_ARG); tracks are then cut
+ *               at such indices only.
  *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums (of the
  *               FP32 block sums, on a region-sorted stream).
  *   FIXED64     deterministic: each run is rounded to a 64-bit fixed-point
@@ -86,8 +89,8 @@ extern "C" {
  *               every FP32 sum of a block of up to 8 segments, against 8x the
  *               bound) is checked against a per-contribution bound that keeps
  *               the sums in range; smoc_run fails with SMOC_ERR_RANGE if one
- *               exceeded it.  Chunked sweeps of a region-sorted stream are
- *               refused as for DET_F64. */
+ *               exceeded it.  Chunk lengths on a region-sorted stream as for
+ *               DET_F64. */
 #define SMOC_TALLY_ATOMIC_F32 0
 #define SMOC_TALLY_ATOMIC_F64 1
 #define SMOC_TALLY_DET_F64 2
@@ -118,7 +121,10 @@ typedef struct {
     int groups_per_lane; /* 1 or 2 energy groups per lane                     */
     int waves_per_block; /* wavefronts per workgroup (1..16; default 4)     */
     int chunk;           /* split tracks longer than this many segments into
-                            chunks joined by an affine psi scan (0 = never)    */
+                            chunks of about this length joined by an affine
+                            psi scan (0 = never); the first chunk of a track
+                            may be up to 1023 segments longer when chunk is
+                            a multiple of 1024                                */
     int persistent;      /* 1: persistent workgroups pulling work from a queue */
     int use_graph;       /* 1: capture the run into a hipGraph, replay it      */
     int autotune;        /* 1: measure candidates on first run, keep the best  */
