@@ -131,9 +131,9 @@ struct smoc_ctx {
     // deterministic tally bookkeeping
     int64_t n_runs = 0;
     DevBuf<int64_t> track_run_off, fsr_run_off, fsr_run_idx;
-    // region-sorted stream: the run id at each track's split points (every
-    // SMOC_RUN_SPLIT segments from its start), track by track; host copies of
-    // it and of track_run_off for the chunk plan
+    // region-sorted stream: the id of the run that begins at each split point
+    // inside a track (a stream index that is a multiple of SMOC_RUN_SPLIT),
+    // track by track; host copies of it and of track_run_off for the chunk plan
     DevBuf<int64_t> split_run;
     std::vector<int64_t> h_track_run_off, h_split_run;
     DevBuf<unsigned char> part;
@@ -974,9 +974,10 @@ struct RunBufs {
 static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *what)
 {
     const bool runs = c->runs && !l.segment_flush;
-    // FIXED64 too: the run sums are what is rounded to fixed point.  Runs end
-    // at every SMOC_RUN_SPLIT-th segment of a track, so chunks of a multiple
-    // of that length split no run, and others would make results depend on
+    // FIXED64 too: the run sums are what is rounded to fixed point.  Runs also
+    // end at split points (stream indices that are multiples of
+    // SMOC_RUN_SPLIT), where chunks of a multiple of that length are cut, so
+    // such chunks split no run; other lengths would make results depend on
     // the chunk length.
     if (l.chunk > 0 && (c->det() || c->fixed()) && runs && l.chunk % SMOC_RUN_SPLIT != 0) {
         smoc_set_error("%s: chunked sweeps of a region-sorted stream with a deterministic "

@@ -693,10 +693,10 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         // segments that all take the series branch have their psi-free parts
         // evaluated together before their psi updates run in order.  A run
         // also ends before every stream index that is a multiple of
-        // SMOC_RUN_SPLIT (the run table of the deterministic tally splits
-        // there too), so a chunk whose length is a multiple of that length,
-        // cut at split points, begins a run and the blocks, and the sums, do
-        // not depend on the chunking.
+        // SMOC_RUN_SPLIT (a split point; the run table of the deterministic
+        // tally splits there too).  Chunks of a multiple of that length are
+        // cut at split points, so each begins a run, and the blocks and the
+        // sums do not depend on the chunking.
         if (kb32 < ke32) {
             RunK rk[GPL];
             uint32_t k = kb32;
