@@ -144,9 +144,13 @@ typedef struct {
 typedef struct {
     double total_ms;   /* whole run, HIP events on the context stream        */
     double reset_ms;   /* fine_flux reset                                    */
-    double xfer_ms;    /* chunked runs: transfer pass + psi scan              */
+    double xfer_ms;    /* chunked runs: transfer pass + psi scan (chunked
+                          deterministic runs of a region-sorted stream, which
+                          sweep every chunk once: the chunk plan only)       */
     double sweep_ms;   /* attenuation sweep kernel (the whole graph with
-                          use_graph, whose phases are not timed apart)        */
+                          use_graph, whose phases are not timed apart; with
+                          the chunked deterministic runs above, also the psi
+                          scan and the per-run correction)                   */
     double reduce_ms;  /* deterministic segmented reduction (DET_* only)     */
     int64_t segments;  /* segments swept                                     */
     int64_t work_items;/* (track or chunk) x group-slice items launched      */

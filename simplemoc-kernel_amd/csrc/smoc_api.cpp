@@ -137,6 +137,9 @@ struct smoc_ctx {
     DevBuf<int64_t> split_run;
     std::vector<int64_t> h_track_run_off, h_split_run;
     DevBuf<unsigned char> part;
+    // region-sorted stream: per run, its tally per unit of incoming psi when
+    // its chunk is swept from psi = 0 (linear chunks) [n_runs][Gp]
+    DevBuf<float> wrun;
 
     // outer (source) iteration
     DevBuf<float> scatter, q_ext;  // [R][G] scattering ratio, [R][F][G] external source
@@ -468,6 +471,7 @@ int smoc_destroy(smoc_ctx *c)
     c->fsr_run_idx.release();
     c->split_run.release();
     c->part.release();
+    c->wrun.release();
     c->queue.release();
     c->plan.release();
     if (c->graph_exec)
@@ -680,6 +684,10 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
         rc = c->split_run.alloc(std::max<size_t>(split.size(), 1));
     if (rc == SMOC_OK)
         rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
+    if (rc == SMOC_OK && c->runs)
+        rc = c->wrun.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp);
+    else if (rc == SMOC_OK)
+        c->wrun.release();
     if (rc != SMOC_OK)
         return rc;
     if (!split.empty())
@@ -875,6 +883,9 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
 // segment, where every segment is a run, or on a region-sorted stream the run
 // that begins at its first segment: C is then a multiple of SMOC_RUN_SPLIT, so
 // each chunk after a track's first begins at one of the track's split points.
+// There the chunks are linear (MODE_LIN): phase 1 sweeps every chunk, those
+// after a track's first from psi = 0, and list2 names the chunks whose runs
+// k_lin_fix corrects.
 static int build_plan(smoc_ctx *c, int C)
 {
     Plan &pl = c->plan;
@@ -911,8 +922,8 @@ static int build_plan(smoc_ctx *c, int C)
                 p1.emplace_back(ce - cb, (n << 2) | (nc > 1 ? 2 : 0));
             else
                 p2.emplace_back(ce - cb, n << 2);
-            if (k > 0 && k + 1 < nc)
-                p1.emplace_back(ce - cb, (n << 2) | 1);
+            if (k > 0 && (split_ids || k + 1 < nc))  // linear chunk (3) or transfer (1)
+                p1.emplace_back(ce - cb, (n << 2) | (split_ids ? 3 : 1));
         }
         sp += smoc_split_count(b, e);
     }
@@ -1085,15 +1096,22 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
                                     c->flux_f64() ? 1 : 0, c->stream));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], c->stream));
+    // linear chunks: a deterministic tally of a region-sorted stream, whose
+    // chunks begin runs (MODE_LIN)
+    const bool linear = chunked && c->det() && runs;
     if (chunked) {
         if (b.plan_chunk > 0)
             LAUNCHCHK(smoc_launch_plan(
                 c->track_off.p, c->T, b.plan_chunk, b.items, b.list1, b.list2,
-                b.track_item_off, c->det() && c->runs ? c->track_run_off.p : nullptr,
-                c->det() && c->runs ? c->split_run.p : nullptr, c->stream));
-        // phase 1: first chunks (swept from psi_in) + transfers of middle chunks
+                b.track_item_off, linear ? c->track_run_off.p : nullptr,
+                linear ? c->split_run.p : nullptr, c->stream));
+        if (linear && ev)  // the sweep time then covers phase 1, scan and correction
+            HIPCHK(hipEventRecord(ev[2], c->stream));
+        // phase 1: first chunks (swept from psi_in) + transfers of middle chunks,
+        // or with linear chunks every chunk
         a.items = b.items;
         a.xfer = b.xfer;
+        a.wrun = c->wrun.p;
         smoc_sweep_args p1 = a;
         p1.xfer_mode = 1;
         p1.item_list = b.list1;
@@ -1107,16 +1125,24 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
         LAUNCHCHK(smoc_launch_sweep(&p1, c->p.storage, gpl, c->p.exp_mode, c->p.tally, wpb,
                                     (int)grid_for(b.n1), c->stream));
         if (b.n2 > 0) {
-            smoc_scan_args sa{b.track_item_off, b.xfer, c->psi_in.p, b.chunk_psi, c->T,
-                              c->Gp, slices};
+            smoc_scan_args sa{b.track_item_off, b.xfer, c->psi_in.p, b.chunk_psi,
+                              linear ? c->psi_out.p : nullptr, c->T, c->Gp, slices};
             LAUNCHCHK(smoc_launch_scan(&sa, gpl, c->stream));
         }
-        if (ev)
-            HIPCHK(hipEventRecord(ev[2], c->stream));
-        // phase 2: the remaining chunks from the scanned psi
-        a.item_list = b.list2;
-        a.chunk_psi = b.chunk_psi;
-        a.n_items = b.n2;
+        if (linear) {
+            // each run of a chunk after a track's first gets its incoming psi's share
+            smoc_fix_args fa{b.items, b.list2, b.chunk_psi, c->wrun.p, c->part.p,
+                             b.n2, b.n_items, c->n_runs, G, c->Gp, slices};
+            LAUNCHCHK(smoc_launch_lin_fix(&fa, c->p.tally, gpl, c->stream));
+            a.n_items = 0;  // no phase 2
+        } else {
+            if (ev)
+                HIPCHK(hipEventRecord(ev[2], c->stream));
+            // phase 2: the remaining chunks from the scanned psi
+            a.item_list = b.list2;
+            a.chunk_psi = b.chunk_psi;
+            a.n_items = b.n2;
+        }
     } else {
         a.n_items = c->T;
         if (ev)
@@ -1436,7 +1462,7 @@ int smoc_run_noalloc(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing
         for (int64_t t = 0; t < c->T; t++) {
             const int64_t nc = smoc_chunk_count(c->h_track_off[t], c->h_track_off[t + 1], C);
             n += nc;
-            n1 += nc > 1 ? nc - 1 : 1;
+            n1 += c->det() && c->runs ? nc : nc > 1 ? nc - 1 : 1;  // linear chunks: all
             n2 += nc - 1;
         }
         const size_t gp = (size_t)c->Gp;

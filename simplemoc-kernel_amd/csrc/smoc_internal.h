@@ -58,6 +58,8 @@ typedef struct {
                                     2: sweep and emit psi_out as a transfer) */
     const float *chunk_psi;      /* [items][Gp] incoming psi of each chunk, or NULL */
     float *xfer;                 /* transfer pass output [items][2][Gp]: B, M */
+    float *wrun;                 /* linear chunks: per run, the tally per unit of
+                                    the chunk's incoming psi [runs][Gp]       */
     int64_t n_items;             /* work entries: tracks, chunks or list length */
     int xfer_mode;               /* 1: phase-1 launch, mode chosen per entry */
     int64_t *queue;              /* persistent mode: work counter, or NULL    */
@@ -76,9 +78,26 @@ typedef struct {
     const float *xfer;             /* [items][2][Gp] */
     const float *psi_in;           /* [T][Gp] */
     float *chunk_psi;              /* [items][Gp] */
+    float *psi_out;                /* linear chunks: [T][Gp] psi_out of the tracks of
+                                      more than one chunk, from the last chunk's
+                                      transfer; NULL otherwise */
     int64_t T;
     int Gp, slices;
 } smoc_scan_args;
+
+/* Correction of linear chunks (k_lin_fix): for every chunk c in list (chunks
+ * swept from psi = 0) and every run r of c, part[r] += w chunk_psi[c] wrun[r]
+ * (w the track weight), so that the run's partial is its tally from the
+ * chunk's true incoming psi. */
+typedef struct {
+    const int64_t *items;          /* [n_items][4]: track, begin, end, first run id */
+    const int64_t *list;           /* chunk ids << 2 */
+    const float *chunk_psi;        /* [items][Gp] */
+    const float *wrun;             /* [runs][Gp] */
+    void *part;                    /* DET_F64: double [runs][Gp]; DET_KAHAN: float2 */
+    int64_t n_list, n_items, n_runs;
+    int G, Gp, slices;
+} smoc_fix_args;
 
 typedef struct {
     const float *sigT;        /* [R][G] FP32 master */
@@ -121,6 +140,7 @@ int smoc_launch_sweep(const smoc_sweep_args *a, int storage, int gpl, int exp_mo
 int smoc_launch_reduce(const smoc_reduce_args *a, int tally, int gpl, int waves_per_block,
                        void *stream);
 int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream);
+int smoc_launch_lin_fix(const smoc_fix_args *a, int tally, int gpl, void *stream);
 int smoc_launch_zero_i64(int64_t *p, int64_t n, void *stream);
 int64_t smoc_update_blocks(int64_t n);
 int smoc_launch_source_update(const smoc_update_args *a, int flux_f64, void *stream);

@@ -417,10 +417,20 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // Chunked runs go in two launches.  Phase 1 sweeps every track's first chunk
 // (its incoming psi is psi_in) and, in the same launch, runs the transfer pass
 // of the middle chunks; phase 2, after the scan, sweeps the remaining chunks.
-// MODE_MIXED kernels pick the mode per work entry from its ITEM_XFER flag.
+// MODE_MIXED kernels pick the mode per work entry from its ITEM_* flags.
+//
+// MODE_LIN (linear chunks; deterministic tallies of a region-sorted stream,
+// whose chunks begin runs): every chunk after a track's first is swept once,
+// in phase 1, from psi = 0.  A segment's tally is affine in the chunk's
+// incoming psi: its psi term is psi_k ev_k / sigT with psi_k = psi0_k +
+// psi_in P_k, P_k the product of 1 - ev over the chunk's earlier segments.
+// The run's partial gets the tally from psi0 and wrun[run] the run's
+// sum of P_k ev_k / sigT; the chunk emits its transfer (psi0_out, P_end).
+// After the scan, k_lin_fix adds psi_in times wrun to each run's partial and
+// k_scan gives the tracks' psi_out.  No segment is swept twice.
 
-enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2 };
-enum { ITEM_XFER = 1, ITEM_EMIT = 2 };
+enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2, MODE_LIN = 3 };
+enum { ITEM_XFER = 1, ITEM_EMIT = 2, ITEM_LIN = 3 };
 
 // Per-lane constants of one run (a track's consecutive segments in one
 // (region, interval)): everything of kernel.c:111-332 that depends on the
@@ -463,7 +473,8 @@ __device__ __forceinline__ RunK run_consts(const float4 t, int kind)
 // of constants regrouped (rounding-level differences only).
 template <int EXPM>
 __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zin,
-                                               float &psi, float &one_m_ev, const ExpTab &tb)
+                                               float &psi, float &one_m_ev, float &ev,
+                                               const ExpTab &tb)
 {
 #pragma clang fp contract(off)
     const float tau = k.sig * ds;
@@ -481,6 +492,7 @@ __device__ __forceinline__ float attenuate_run(const RunK &k, float ds, float zi
     psi = __builtin_fmaf(q0s, x.ev, __builtin_fmaf(q1m, m2, __builtin_fmaf(k.q2m, reuse,
                                                                           psi * x.one_m_ev)));
     one_m_ev = x.one_m_ev;
+    ev = x.ev;
     return flux_integral;
 }
 
@@ -590,15 +602,19 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
     // compute on it and never flush; their psi stores land in padding.
     const int Gp = a.Gp, G = a.G;
     int g[GPL], gi[GPL];
+    // mprod: the running product of 1 - ev (MODE_XFER, MODE_LIN); wacc: the
+    // run's sum of mprod ev / sigT (MODE_LIN)
     float psi[GPL], mprod[GPL];
+    double wacc[GPL];
     Acc<TALLY, !RUNS> acc[GPL];
     const float *pin = a.chunk_psi ? a.chunk_psi + it * Gp : a.psi_in + t * Gp;
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         g[j] = sl * 64 * GPL + j * 64 + lane;
         gi[j] = g[j] < G ? g[j] : G - 1;
-        psi[j] = MODE == MODE_XFER ? 0.f : pin[gi[j]];
+        psi[j] = MODE == MODE_XFER || MODE == MODE_LIN ? 0.f : pin[gi[j]];
         mprod[j] = 1.f;
+        wacc[j] = 0.0;
         acc_init(acc[j], a.fix_limit);
     }
 
@@ -778,12 +794,13 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
-                                // block sums: series_a's three psi-free parts and
-                                // series_b's sum of psi ev
-                                float b0[GPL], bs[GPL], s2[GPL], ps[GPL];
+                                // block sums: series_a's three psi-free parts,
+                                // series_b's sum of psi ev and (MODE_LIN) the
+                                // sum of mprod ev
+                                float b0[GPL], bs[GPL], s2[GPL], ps[GPL], ws[GPL];
 #pragma unroll
                                 for (int j = 0; j < GPL; j++)
-                                    b0[j] = bs[j] = s2[j] = ps[j] = 0.f;
+                                    b0[j] = bs[j] = s2[j] = ps[j] = ws[j] = 0.f;
 #pragma unroll
                                 for (int u = 0; u < N; u++) {
                                     const RawSeg &sg = u == 0 ? cur : nx[u - 1];
@@ -800,6 +817,10 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                     for (int u = 0; u < N; u++) {
                                         if constexpr (MODE == MODE_XFER)
                                             mprod[j] *= 1.f - a[u][j].ev;
+                                        if constexpr (MODE == MODE_LIN) {
+                                            ws[j] = __builtin_fmaf(mprod[j], a[u][j].ev, ws[j]);
+                                            mprod[j] = __builtin_fmaf(-mprod[j], a[u][j].ev, mprod[j]);
+                                        }
                                         series_b(a[u][j], psi[j], ps[j]);
                                     }
                                     // the block's flux integrals, summed in FP32, join
@@ -811,6 +832,8 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                             t = __builtin_fmaf(s2[j], rk[j].q2m, t);
                                         t = __builtin_fmaf(b0[j], rk[j].sig, t);
                                         acc_add_block(acc[j], __builtin_fmaf(ps[j], rk[j].i1, t), N);
+                                        if constexpr (MODE == MODE_LIN)
+                                            wacc[j] += (double)(ws[j] * rk[j].i1);
                                     }
                                 }
                                 k += N;
@@ -828,13 +851,18 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                         } else {
 #pragma unroll
                             for (int j = 0; j < GPL; j++) {
-                                float m;
+                                float m, ev;
                                 const float fi = attenuate_run<EXPM>(rk[j], cur.len, cur.zin,
-                                                                     psi[j], m, tb);
-                                if constexpr (MODE == MODE_XFER)
+                                                                     psi[j], m, ev, tb);
+                                if constexpr (MODE == MODE_XFER) {
                                     mprod[j] *= m;
-                                else
+                                } else {
                                     acc[j].add(fi);
+                                    if constexpr (MODE == MODE_LIN) {
+                                        wacc[j] += (double)((mprod[j] * ev) * rk[j].i1);
+                                        mprod[j] *= m;
+                                    }
+                                }
                             }
                             ++k;
                             cur = nx[0];
@@ -847,7 +875,15 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                     walk(std::true_type{});
                 else
                     walk(std::false_type{});
-                if constexpr (MODE == MODE_SWEEP)
+                if constexpr (MODE == MODE_LIN) {
+#pragma unroll
+                    for (int j = 0; j < GPL; j++) {
+                        if (g[j] < G)
+                            a.wrun[run * Gp + g[j]] = (float)wacc[j];
+                        wacc[j] = 0.0;
+                    }
+                }
+                if constexpr (MODE == MODE_SWEEP || MODE == MODE_LIN)
                     flush_all(ccode >> 2);
                 if (k >= ke32)
                     break;
@@ -855,14 +891,14 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         }
     }
 
-    if (MODE == MODE_XFER || emit_xfer) {
+    if (MODE == MODE_XFER || MODE == MODE_LIN || emit_xfer) {
         // transfer of this chunk; a swept first chunk knows its psi_out
         // exactly and passes it on as B with M = 0
         float *xb = a.xfer + (2 * it) * Gp, *xm = a.xfer + (2 * it + 1) * Gp;
 #pragma unroll
         for (int j = 0; j < GPL; j++) {
             xb[g[j]] = psi[j];
-            xm[g[j]] = MODE == MODE_XFER ? mprod[j] : 0.f;
+            xm[g[j]] = MODE == MODE_XFER || MODE == MODE_LIN ? mprod[j] : 0.f;
         }
     } else if (!items || ke == uniform64(track_off[t + 1])) {
         // chunks other than a track's last carry no psi_out
@@ -884,12 +920,19 @@ __device__ __forceinline__ void dispatch_item(const smoc_sweep_args &a,
 {
     if constexpr (MODE == MODE_MIXED) {
         const int64_t w = a.slices == 1 ? item : item / a.slices;
-        if (uniform64(a.item_list[w]) & ITEM_XFER)
+        const int flags = (int)(uniform64(a.item_list[w]) & 3);
+        constexpr bool LIN = RUNS && (TALLY == SMOC_TALLY_DET_F64 || TALLY == SMOC_TALLY_DET_KAHAN);
+        if (LIN && flags == ITEM_LIN) {
+            if constexpr (LIN)
+                sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_LIN>(a, len, code, zin, track_off,
+                                                                    items, item, lane, tb);
+        } else if (flags == ITEM_XFER) {
             sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_XFER>(a, len, code, zin, track_off,
                                                                  items, item, lane, tb);
-        else
+        } else {
             sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_SWEEP>(a, len, code, zin, track_off,
                                                                   items, item, lane, tb);
+        }
     } else {
         sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE>(a, len, code, zin, track_off, items,
                                                         item, lane, tb);
@@ -965,7 +1008,9 @@ __global__ void __launch_bounds__(1024)
 }
 
 // Scan of chunk transfers along each track: chunk_psi[first chunk] = psi_in,
-// chunk_psi[c+1] = B_c + M_c chunk_psi[c].  One wave per (track, slice).
+// chunk_psi[c+1] = B_c + M_c chunk_psi[c]; with linear chunks, the psi_out
+// of a track of several chunks is B + M psi_in of its last chunk.  One wave
+// per (track, slice).
 template <int GPL>
 __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
 {
@@ -987,6 +1032,52 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
             if (c + 1 < ce)
                 psi = a.xfer[(2 * c) * Gp + g] + a.xfer[(2 * c + 1) * Gp + g] * psi;
         }
+        if (a.psi_out && ce - cb > 1)
+            a.psi_out[t * Gp + g] = a.xfer[(2 * (ce - 1)) * Gp + g] +
+                                    a.xfer[(2 * (ce - 1) + 1) * Gp + g] * psi;
+    }
+}
+
+// Linear chunks: each run r of a chunk c swept from psi = 0 gets w psi_in(c)
+// wrun[r] added to its partial (w the track weight), in the partial's own
+// arithmetic (FP64, or a compensated FP32 step).  One wave per (list entry,
+// slice); every run belongs to one chunk, so each partial is written once.
+template <int TALLY, int GPL>
+__global__ void __launch_bounds__(256) k_lin_fix(smoc_fix_args a)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= a.n_list * a.slices)
+        return;
+    const int64_t w = item / a.slices;
+    const int sl = (int)(item - w * a.slices);
+    const int Gp = a.Gp;
+    const int64_t c = uniform64(a.list[w]) >> 2;
+    const int64_t r0 = uniform64(a.items[4 * c + 3]);
+    const int64_t r1 = c + 1 < a.n_items ? uniform64(a.items[4 * (c + 1) + 3]) : a.n_runs;
+#pragma unroll
+    for (int j = 0; j < GPL; j++) {
+        const int g = sl * 64 * GPL + j * 64 + lane;
+        if (g >= a.G)
+            continue;
+        const float pin = a.chunk_psi[c * Gp + g];
+        for (int64_t r = r0; r < r1; ++r) {
+            const float wr = a.wrun[r * Gp + g];
+            if constexpr (TALLY == SMOC_TALLY_DET_F64) {
+                double *p = static_cast<double *>(a.part) + r * Gp + g;
+                *p = *p + (double)SMOC_WEIGHT * ((double)pin * (double)wr);
+            } else {
+                float2 *p = static_cast<float2 *>(a.part) + r * Gp + g;
+                float2 v = *p;  // (s, c): s - c, as the run accumulator keeps it
+                const float y = SMOC_WEIGHT * (pin * wr) - v.y;
+                const float s = v.x + y;
+                v.y = (s - v.x) - y;
+                v.x = s;
+                *p = v;
+            }
+        }
     }
 }
 
@@ -1001,11 +1092,12 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
 // a multiple of SMOC_RUN_SPLIT, so every chunk after a track's first begins at
 // a split point), track_run_off[t] for the first chunk and the split_run entry
 // of the chunk's split point for the others; split_run holds, track by track,
-// the run id at each of the track's smoc_split_count split points.  These are
-// the items of the host plan; the lists are in track order.  One workgroup of
-// 1024 threads: thread i takes a contiguous range of tracks, counts its
-// chunks, list entries and split points, a scan over the threads gives its
-// offsets, then it writes.
+// the run id at each of the track's smoc_split_count split points.  With the
+// run tables the chunks are linear (MODE_LIN): list1 then holds every chunk,
+// those after a track's first with ITEM_LIN.  These are the items of the host
+// plan; the lists are in track order.  One workgroup of 1024 threads: thread i
+// takes a contiguous range of tracks, counts its chunks, list entries and
+// split points, a scan over the threads gives its offsets, then it writes.
 __global__ void __launch_bounds__(1024)
     k_plan(const int64_t *__restrict__ track_off, int64_t T, int C, int64_t *__restrict__ items,
            int64_t *__restrict__ list1, int64_t *__restrict__ list2, int64_t *__restrict__ tio,
@@ -1017,11 +1109,13 @@ __global__ void __launch_bounds__(1024)
     const int64_t t0 = min(T, (int64_t)tid * per), t1 = min(T, t0 + per);
     auto chunks = [&](int64_t t) { return smoc_chunk_count(track_off[t], track_off[t + 1], C); };
     auto splits = [&](int64_t t) { return smoc_split_count(track_off[t], track_off[t + 1]); };
+    // linear chunks (given the run tables): every chunk in list1
+    const bool lin = split_run != nullptr;
     int64_t n = 0, n1 = 0, n2 = 0, ns = 0;
     for (int64_t t = t0; t < t1; ++t) {
         const int64_t nc = chunks(t);
         n += nc;
-        n1 += nc > 1 ? nc - 1 : 1;
+        n1 += lin ? nc : nc > 1 ? nc - 1 : 1;
         n2 += nc - 1;
         ns += splits(t);
     }
@@ -1060,8 +1154,8 @@ __global__ void __launch_bounds__(1024)
                                : split_run[sp + smoc_split_index(b, cb) - 1];
         }
         list1[p1++] = (i << 2) | (nc > 1 ? ITEM_EMIT : 0);
-        for (int64_t k = 1; k + 1 < nc; ++k)
-            list1[p1++] = ((i + k) << 2) | ITEM_XFER;
+        for (int64_t k = 1; k < (lin ? nc : nc - 1); ++k)
+            list1[p1++] = ((i + k) << 2) | (lin ? ITEM_LIN : ITEM_XFER);
         for (int64_t k = 1; k < nc; ++k)
             list2[p2++] = (i + k) << 2;
         i += nc;
@@ -1666,6 +1760,29 @@ extern "C" int smoc_launch_scan(const smoc_scan_args *a, int gpl, void *stream)
         hipLaunchKernelGGL(k_scan<2>, dim3(nb), dim3(256), 0, s, *a);
     else
         hipLaunchKernelGGL(k_scan<1>, dim3(nb), dim3(256), 0, s, *a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int smoc_launch_lin_fix(const smoc_fix_args *a, int tally, int gpl, void *stream)
+{
+    const int64_t n = a->n_list * a->slices;
+    if (n == 0)
+        return 0;
+    if (tally != SMOC_TALLY_DET_F64 && tally != SMOC_TALLY_DET_KAHAN)
+        return (int)hipErrorInvalidValue;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const unsigned nb = blocks_for(n, 4);
+    if (tally == SMOC_TALLY_DET_F64) {
+        if (gpl == 2)
+            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_F64, 2>), dim3(nb), dim3(256), 0, s, *a);
+        else
+            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_F64, 1>), dim3(nb), dim3(256), 0, s, *a);
+    } else {
+        if (gpl == 2)
+            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_KAHAN, 2>), dim3(nb), dim3(256), 0, s, *a);
+        else
+            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_KAHAN, 1>), dim3(nb), dim3(256), 0, s, *a);
+    }
     return (int)hipGetLastError();
 }
 

@@ -462,3 +462,36 @@ def test_sorted_block_boundaries(gpu_ready, seed):
         with pytest.raises(smoc.SmocError) as e:
             ctx.run(chunk=16)
         assert e.value.code == smoc.ERR_ARG
+
+
+def _sorted_lognormal(seed, stretch):
+    """Lognormal tracks sorted over few regions (runs of a few hundred
+    segments); with stretch, every 7th segment 30 times longer: tau up to 30,
+    so those segments leave the series path."""
+    prob = smoc.generate(0, G=100, R=40, F=5, seed=seed, lognormal_tracks=24, median=900.0,
+                         sigma=1.0, lo=16, hi=6000, sorted_stream=True)
+    if stretch:
+        prob.len = prob.len.copy()
+        prob.len[::7] *= np.float32(30.0)
+    return prob
+
+
+@pytest.mark.parametrize("exp", ["accurate", "native", "table"])
+@pytest.mark.parametrize("tally", ["det-f64", "det-kahan"])
+def test_sorted_linear_chunks_general_path(gpu_ready, exp, tally):
+    """Chunks of a region-sorted stream with a deterministic tally are swept
+    once from psi = 0 and corrected per run after the scan: through the
+    general segment path (stretched segments with the accurate exp; every
+    segment with the native and table exps), against the FP64 oracle and
+    bit-exact across launch shapes."""
+    prob = _sorted_lognormal(33, stretch=exp == "accurate")
+    print(f"case linear chunks {exp} {tally}: S {prob.S}", flush=True)
+    ref = None
+    for launch in ({"chunk": 1024}, {"chunk": 1024, "persistent": 1, "groups_per_lane": 1,
+                                     "waves_per_block": 2}):
+        flux, psi, tm = run_gpu(prob, "f32", exp, tally, **launch)
+        assert tm.used.chunk == 1024
+        if ref is None:
+            check_vs_oracle(prob, flux, psi, exp)
+            ref = (flux, psi)
+        assert np.array_equal(flux, ref[0]) and np.array_equal(psi, ref[1]), launch
