@@ -41,6 +41,12 @@ says so, timed over the same number of steps):
     SQ_INSTS_VALU per launch x 64 over the step time) against the FP32 issue
     ceiling of 7.86e13 (157.3 TFLOP/s of FMAs), with `useful_frac` counting
     only the lanes that carry one of the G groups;
+  * `batched_plain` / `batched_mixed`: 8192 track batches of lognormal length
+    (median 4000, sigma 1.0, clipped to [16, 200000], about 5.3e7 segments),
+    the plain kernel on them in generation order (atomic FP32) and the mixed
+    path on the same batches sorted (deterministic FP64, long tracks in
+    chunks), each autotuned and replayed from a hipGraph; `vs_plain` is the
+    ratio of attenuations/s to the metric's plain line;
   * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
     the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
@@ -101,6 +107,7 @@ def parse():
     ap.add_argument("--gpl", type=int, default=0)
     ap.add_argument("--wpb", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-sample-segments", type=int, default=16_000_000)
@@ -426,6 +433,49 @@ def main():
                     "on the sorted stream -> run sums -> deterministic FP64 tally + autotuned "
                     "graph (= mixed); and the mixed configuration on the unsorted stream"}
         mixed = {"prob": mprob, "sorted": sorted_runs, "track_order": (tf, tp)}
+
+    if not args.no_batched:
+        # 8192 track batches of lognormal length (median 4000, sigma 1.0, clipped to
+        # [16, 200000]), about 5.3e7 segments: the plain kernel on them as generated
+        # and the mixed path on the same batches sorted, each autotuned (chunk length,
+        # persistence, groups per lane, waves per workgroup) and replayed from a graph
+        t0 = time.time()
+        bprob = smoc.generate(0, G=G, R=R, F=F, seed=42, lognormal_tracks=8192,
+                              median=4000.0, sigma=1.0, lo=16, hi=200_000)
+        bsorted, bsort_s = smoc.sort_stream(bprob)
+        bgen_s = time.time() - t0
+        tuned = {"autotune": 1, "use_graph": 1}
+        bsg = bprob.S * G
+
+        def bline(problem, tally, workload):
+            log(f"batched: {problem.S} segments in {problem.T} tracks, {tally}")
+            with smoc.Context(G, R, F, storage="f32", exp=args.exp, tally=tally) as ctx:
+                ctx.load(problem)
+                lwall, lsweep, ltotal, lused = timed_steps(ctx, args.steps,
+                                                           max(args.warmup, 1), tuned)
+                lf, lp = ctx.results()
+            v = bsg / (lwall / args.steps)
+            return {"workload": workload, "value": v, "ms_per_step": lwall / args.steps * 1e3,
+                    "vs_plain": v / value, "sweep_ms": lsweep, "total_ms": ltotal,
+                    "segments": problem.S, "tracks": problem.T, "tally": tally,
+                    "launch": lused,
+                    "checksums": dict(zip(("fine_flux", "state_flux"),
+                                          smoc.checksums(lf, lp)))}
+
+        out["batched_plain"] = bline(
+            bprob, "atomic-f32",
+            f"8192 lognormal track batches ({bprob.S} segments) x {G} groups, FP32, tracks "
+            "in generation order, atomic FP32 tally; autotuned launch replayed from a hipGraph")
+        out["batched_mixed"] = bline(
+            bsorted, "det-f64",
+            "the same batches, stream pre-sorted by (region, fine interval); FP32 storage and "
+            "math, runs summed in registers, deterministic FP64 segmented reduction; long "
+            "tracks in chunks swept once each (linear in their incoming psi, corrected per "
+            "run); autotuned launch replayed from a hipGraph")
+        out["batched_mixed"]["vs_batched_plain"] = (out["batched_mixed"]["value"] /
+                                                    out["batched_plain"]["value"])
+        out["batched_input"] = {"generate_and_sort_s": bgen_s, "sort_host_s": bsort_s,
+                                "note": "outside every timed region"}
 
     if not args.no_cpu_baseline:
         log("CPU baseline and FP64 oracle")
