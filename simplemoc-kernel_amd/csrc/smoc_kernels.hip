@@ -719,6 +719,10 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
             RawSeg cur = load_raw(len, code, zin, k);
             for (;;) {
                 const int ccode = cur.code;
+                // the run ends by rend at the latest: the item's end or the next
+                // split point (one min per run: capping each block by both would
+                // compile to a vector min3 of scalars)
+                const uint32_t rend = min(ke32, (k | (SMOC_RUN_SPLIT - 1)) + 1);
                 {
                     const raw_t *__restrict__ base = tup + (int64_t)(ccode >> 2) * Gp;
                     raw_t b[GPL];
@@ -788,9 +792,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                                        series_ok(nx[n - 1]))
                                     ++n;
                             }
-                            // segments k .. k + n - 1 are the item's and
-                            // cross no split point
-                            n = min(n, min(ke32 - k, SMOC_RUN_SPLIT - (k & (SMOC_RUN_SPLIT - 1))));
+                            n = min(n, rend - k);  // segments k .. k + n - 1 are the run's
                             auto block = [&](auto cnt) {
                                 constexpr int N = decltype(cnt)::value;
                                 SegA a[N][GPL];
@@ -867,7 +869,7 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
                             ++k;
                             cur = nx[0];
                         }
-                        if (k >= ke32 || (k & (SMOC_RUN_SPLIT - 1)) == 0 || cur.code != ccode)
+                        if (k >= rend || cur.code != ccode)
                             break;
                     }
                 };
