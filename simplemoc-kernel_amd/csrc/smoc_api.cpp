@@ -138,7 +138,8 @@ struct smoc_ctx {
     std::vector<int64_t> h_track_run_off, h_split_run;
     DevBuf<unsigned char> part;
     // region-sorted stream: per run, its tally per unit of incoming psi when
-    // its chunk is swept from psi = 0 (linear chunks) [n_runs][Gp]
+    // its chunk is swept from psi = 0 (linear chunks) [n_runs][Gp]; allocated
+    // when a track is long enough to be chunked
     DevBuf<float> wrun;
 
     // outer (source) iteration
@@ -684,7 +685,12 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
         rc = c->split_run.alloc(std::max<size_t>(split.size(), 1));
     if (rc == SMOC_OK)
         rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
-    if (rc == SMOC_OK && c->runs)
+    // linear chunks need wrun; a stream can be chunked only when a track is
+    // longer than the chunk length, which is then at least SMOC_RUN_SPLIT
+    int64_t longest = 0;
+    for (int64_t t = 0; t < T; t++)
+        longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
+    if (rc == SMOC_OK && c->runs && longest > SMOC_RUN_SPLIT)
         rc = c->wrun.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp);
     else if (rc == SMOC_OK)
         c->wrun.release();
@@ -996,6 +1002,12 @@ static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *wha
                        "(got %d)",
                        what, SMOC_RUN_SPLIT, l.chunk);
         return SMOC_ERR_ARG;
+    }
+    // linear chunks need wrun, allocated for every stream that can be chunked
+    // (a track longer than the chunk length, so longer than SMOC_RUN_SPLIT)
+    if (l.chunk > 0 && c->det() && runs && !c->wrun.p) {
+        smoc_set_error("%s: no per-run buffer for chunks of this stream", what);
+        return SMOC_ERR_STATE;
     }
     if (l.segment_flush && c->det() && c->runs) {
         smoc_set_error("%s: segment_flush on a region-sorted stream needs an atomic or "
