@@ -409,7 +409,7 @@ int main(int argc, char *argv[])
     printf("%-25s%.6f seconds\n", "Runtime:", mean_s);
     printf("%-25s%.6f ns\n", "Time per Intersection:", tpi);
     printf("%-25s%.4e\n", "Attenuations/s:", (double)S * G / mean_s);
-    printf("%-25s%.3f ms (transfer+scan %.3f, sweep %.3f, reduce %.3f)\n", "Best run:",
+    printf("%-25s%.3f ms (chunk plan/transfer/scan %.3f, sweep %.3f, reduce %.3f)\n", "Best run:",
            best.total_ms, best.xfer_ms, best.sweep_ms, best.reduce_ms);
     printf("%-25sgpl %d, wpb %d, chunk %d, persistent %d, graph %d, reduce wpb %d\n", "Launch:",
            best.used.groups_per_lane, best.used.waves_per_block, best.used.chunk,
