@@ -162,3 +162,25 @@ def test_fixed64_reports_a_diverging_iteration(gpu_ready):
         with pytest.raises(smoc.SmocError) as e:
             ctx.iterate(3)
         assert e.value.code == smoc.ERR_RANGE
+
+
+@pytest.mark.parametrize("tally", ["det-f64", "det-kahan"])
+def test_iterations_with_linear_chunks(gpu_ready, tally):
+    """Outer iterations on a region-sorted stream of long tracks in chunks
+    (each swept once and corrected per run after the scan), replayed from a
+    hipGraph: against the iterated FP64 oracle."""
+    prob = smoc.generate(0, G=100, R=60, F=5, seed=43, lognormal_tracks=32, median=1500.0,
+                         sigma=1.0, lo=16, hi=8000, sorted_stream=True)
+    sc = scattering(prob, 0.8, 2)
+    print(f"case outer iteration x3, linear chunks, {tally}: S {prob.S}", flush=True)
+    with smoc.Context(100, 60, 5, tally=tally) as ctx:
+        ctx.load(prob)
+        ctx.set_scattering(sc)
+        tm, ums, chg = ctx.iterate(3, chunk=1024, persistent=1, use_graph=1)
+        assert tm.used.chunk == 1024
+        flux, psi = ctx.results()
+        src = ctx.source()
+    of, op, osrc = oracle_iterate(prob, sc, prob.fine_source, 3)
+    e = (normwise(flux, of), normwise(psi, op), normwise(src, osrc))
+    print(f"  normwise fine_flux {e[0]:.2e} psi {e[1]:.2e} source {e[2]:.2e}", flush=True)
+    assert max(e) <= TOL and chg > 0
