@@ -1025,18 +1025,38 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
     const int sl = (int)(item - t * a.slices);
     const int Gp = a.Gp;
     const int64_t cb = a.track_item_off[t], ce = a.track_item_off[t + 1];
+    // the outputs do not overlap the transfers, so the loads of a group of
+    // chunks go out together ahead of the chain of FMAs (a long track has
+    // about a hundred chunks)
+    const float *__restrict__ xf = a.xfer;
+    float *__restrict__ cpsi = a.chunk_psi;
 #pragma unroll
     for (int j = 0; j < GPL; j++) {
         const int g = sl * 64 * GPL + j * 64 + lane;
         float psi = a.psi_in[t * Gp + g];
-        for (int64_t c = cb; c < ce; ++c) {
-            a.chunk_psi[c * Gp + g] = psi;
+        int64_t c = cb;
+        for (; c + 8 <= ce; c += 8) {
+            float bv[8], mv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                bv[u] = xf[(2 * (c + u)) * Gp + g];
+                mv[u] = xf[(2 * (c + u) + 1) * Gp + g];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                cpsi[(c + u) * Gp + g] = psi;
+                if (c + u + 1 < ce)
+                    psi = bv[u] + mv[u] * psi;
+            }
+        }
+        for (; c < ce; ++c) {
+            cpsi[c * Gp + g] = psi;
             if (c + 1 < ce)
-                psi = a.xfer[(2 * c) * Gp + g] + a.xfer[(2 * c + 1) * Gp + g] * psi;
+                psi = xf[(2 * c) * Gp + g] + xf[(2 * c + 1) * Gp + g] * psi;
         }
         if (a.psi_out && ce - cb > 1)
-            a.psi_out[t * Gp + g] = a.xfer[(2 * (ce - 1)) * Gp + g] +
-                                    a.xfer[(2 * (ce - 1) + 1) * Gp + g] * psi;
+            a.psi_out[t * Gp + g] = xf[(2 * (ce - 1)) * Gp + g] +
+                                    xf[(2 * (ce - 1) + 1) * Gp + g] * psi;
     }
 }
 
