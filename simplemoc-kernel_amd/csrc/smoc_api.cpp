@@ -1334,30 +1334,38 @@ static int autotune(smoc_ctx *c, const smoc_launch &base, smoc_launch *best_out)
         if ((rc = consider(cand)) != SMOC_OK)
             return rc;
     }
-    if (c->max_track > 4 * std::max<int64_t>(mean_len, 1)) {
-        const smoc_launch base_split = best;
-        for (int C : {1024, 2048, 4096, 8192, 16384}) {
-            for (int pers = 0; pers < 2; pers++) {
-                smoc_launch cand = base_split;
-                cand.chunk = C;
-                cand.seg_per_block = 0;
-                cand.persistent = pers;
-                if ((rc = consider(cand)) != SMOC_OK)
-                    return rc;
+    // coordinate search over chunk length (with persistence), groups per lane
+    // and waves per workgroup; a second pass when the first changed anything,
+    // since the best chunk length can depend on the workgroup shape
+    for (int pass = 0; pass < 2; pass++) {
+        const smoc_launch before = best;
+        if (c->max_track > 4 * std::max<int64_t>(mean_len, 1)) {
+            const smoc_launch base_split = best;
+            for (int C : {1024, 2048, 4096, 8192, 16384}) {
+                for (int pers = 0; pers < 2; pers++) {
+                    smoc_launch cand = base_split;
+                    cand.chunk = C;
+                    cand.seg_per_block = 0;
+                    cand.persistent = pers;
+                    if ((rc = consider(cand)) != SMOC_OK)
+                        return rc;
+                }
             }
         }
-    }
-    if (c->p.egroups > 64) {
-        smoc_launch cand = best;
-        cand.groups_per_lane = 3 - best.groups_per_lane;
-        if ((rc = consider(cand)) != SMOC_OK)
-            return rc;
-    }
-    for (int w : {1, 2, 4, 8, 16}) {
-        smoc_launch cand = best;
-        cand.waves_per_block = w;
-        if ((rc = consider(cand)) != SMOC_OK)
-            return rc;
+        if (c->p.egroups > 64) {
+            smoc_launch cand = best;
+            cand.groups_per_lane = 3 - best.groups_per_lane;
+            if ((rc = consider(cand)) != SMOC_OK)
+                return rc;
+        }
+        for (int w : {1, 2, 4, 8, 16}) {
+            smoc_launch cand = best;
+            cand.waves_per_block = w;
+            if ((rc = consider(cand)) != SMOC_OK)
+                return rc;
+        }
+        if (same_launch(best, before))
+            break;
     }
     if (c->det()) {  // the reduction's tile: waves per workgroup
         for (int w : {1, 2, 8, 16}) {
