@@ -1077,6 +1077,9 @@ __global__ void __launch_bounds__(256) k_lin_fix(smoc_fix_args a)
     const int sl = (int)(item - w * a.slices);
     const int Gp = a.Gp;
     const int64_t c = uniform64(a.list[w]) >> 2;
+    // items are in track order and a track's chunks in stream order, and runs
+    // are numbered the same way, so chunk c's runs end where the next item's
+    // begin (the last item's at n_runs)
     const int64_t r0 = uniform64(a.items[4 * c + 3]);
     const int64_t r1 = c + 1 < a.n_items ? uniform64(a.items[4 * (c + 1) + 3]) : a.n_runs;
 #pragma unroll
