@@ -79,6 +79,7 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // Chunking of long tracks for one chunk length C.
 struct Plan {
     int chunk = 0;
+    bool linear = false;  // linear chunks (MODE_LIN entries, run ids from the tables)
     int64_t n_items = 0, n1 = 0, n2 = 0;
     DevBuf<int64_t> items;           // [n_items][4]
     DevBuf<int64_t> list1, list2;    // phase-1 / phase-2 work entries
@@ -94,6 +95,7 @@ struct Plan {
         chunk_psi.release();
         xfer.release();
         chunk = 0;
+        linear = false;
         n_items = n1 = n2 = 0;
     }
 };
@@ -141,6 +143,8 @@ struct smoc_ctx {
     // its chunk is swept from psi = 0 (linear chunks) [n_runs][Gp]; allocated
     // when a track is long enough to be chunked
     DevBuf<float> wrun;
+    // atomic tallies on such a stream: the (region, interval) row of each run
+    DevBuf<int32_t> run_row;
 
     // outer (source) iteration
     DevBuf<float> scatter, q_ext;  // [R][G] scattering ratio, [R][F][G] external source
@@ -473,6 +477,7 @@ int smoc_destroy(smoc_ctx *c)
     c->split_run.release();
     c->part.release();
     c->wrun.release();
+    c->run_row.release();
     c->queue.release();
     c->plan.release();
     if (c->graph_exec)
@@ -616,19 +621,39 @@ int smoc_set_regions(smoc_ctx *c, const float *sigT, const float *fine_source,
     return SMOC_OK;
 }
 
-// Run bookkeeping of the deterministic tally: runs are numbered track by
-// track in stream order; when c->runs a run is a maximal stretch of one
-// track's segments with the same (region, interval) that does not cross a
-// split point (a stream index that is a multiple of SMOC_RUN_SPLIT: the sweep
-// ends its runs there too), else a single segment.  fsr_run_idx lists, for
-// every (region, interval), its runs in ascending id; split_run the id of the
-// run that begins at each split point inside a track, track by track.
+// Run bookkeeping: runs are numbered track by track in stream order; when
+// c->runs a run is a maximal stretch of one track's segments with the same
+// (region, interval) that does not cross a split point (a stream index that
+// is a multiple of SMOC_RUN_SPLIT: the sweep ends its runs there too), else a
+// single segment.  split_run holds the id of the run that begins at each
+// split point inside a track, track by track.  The deterministic tallies get
+// all of it: fsr_run_idx lists, for every (region, interval), its runs in
+// ascending id, and part holds the partials.  An atomic FP32/FP64 tally on a
+// sorted stream that can be chunked (a track longer than SMOC_RUN_SPLIT) gets
+// the track and split-point tables, run_row and wrun, for linear chunks.
 static_assert((SMOC_RUN_SPLIT & (SMOC_RUN_SPLIT - 1)) == 0, "split length: a power of two");
 static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float *z, int64_t T,
                       const int64_t *track_off)
 {
     const int F = c->p.fine_axial;
     const int64_t K = c->n_fsr();
+    const bool det = c->det();
+    // linear chunks need wrun; a stream can be chunked only when a track is
+    // longer than the chunk length, which is then at least SMOC_RUN_SPLIT
+    int64_t longest = 0;
+    for (int64_t t = 0; t < T; t++)
+        longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
+    const bool chunkable = c->runs && longest > SMOC_RUN_SPLIT;
+    if (!det && !chunkable) {
+        c->track_run_off.release();
+        c->split_run.release();
+        c->wrun.release();
+        c->run_row.release();
+        c->h_track_run_off.clear();
+        c->h_split_run.clear();
+        c->n_runs = 0;
+        return SMOC_OK;
+    }
     std::vector<int32_t> run_key;
     std::vector<int64_t> tro((size_t)T + 1), split;
     try {
@@ -666,31 +691,10 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     if (c->runs)
         c->h_track_run_off[T] = tro[T];
     const int64_t n_runs = tro[T];
-    std::vector<int64_t> fro((size_t)K + 1, 0), idx((size_t)std::max<int64_t>(n_runs, 1));
-    for (int64_t r = 0; r < n_runs; r++)
-        fro[run_key[r] + 1]++;
-    for (int64_t k = 0; k < K; k++)
-        fro[k + 1] += fro[k];
-    {
-        std::vector<int64_t> pos(fro.begin(), fro.end() - 1);
-        for (int64_t r = 0; r < n_runs; r++)
-            idx[pos[run_key[r]]++] = r;
-    }
     int rc = c->track_run_off.alloc((size_t)T + 1);
     if (rc == SMOC_OK)
-        rc = c->fsr_run_off.alloc((size_t)K + 1);
-    if (rc == SMOC_OK)
-        rc = c->fsr_run_idx.alloc((size_t)std::max<int64_t>(n_runs, 1));
-    if (rc == SMOC_OK)
         rc = c->split_run.alloc(std::max<size_t>(split.size(), 1));
-    if (rc == SMOC_OK)
-        rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
-    // linear chunks need wrun; a stream can be chunked only when a track is
-    // longer than the chunk length, which is then at least SMOC_RUN_SPLIT
-    int64_t longest = 0;
-    for (int64_t t = 0; t < T; t++)
-        longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
-    if (rc == SMOC_OK && c->runs && longest > SMOC_RUN_SPLIT)
+    if (rc == SMOC_OK && chunkable)
         rc = c->wrun.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp);
     else if (rc == SMOC_OK)
         c->wrun.release();
@@ -701,10 +705,38 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
                          hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->track_run_off.p, tro.data(), sizeof(int64_t) * (T + 1),
                      hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->fsr_run_off.p, fro.data(), sizeof(int64_t) * (K + 1),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->fsr_run_idx.p, idx.data(),
-                     sizeof(int64_t) * std::max<int64_t>(n_runs, 1), hipMemcpyHostToDevice));
+    if (det) {
+        std::vector<int64_t> fro((size_t)K + 1, 0), idx((size_t)std::max<int64_t>(n_runs, 1));
+        for (int64_t r = 0; r < n_runs; r++)
+            fro[run_key[r] + 1]++;
+        for (int64_t k = 0; k < K; k++)
+            fro[k + 1] += fro[k];
+        {
+            std::vector<int64_t> pos(fro.begin(), fro.end() - 1);
+            for (int64_t r = 0; r < n_runs; r++)
+                idx[pos[run_key[r]]++] = r;
+        }
+        rc = c->fsr_run_off.alloc((size_t)K + 1);
+        if (rc == SMOC_OK)
+            rc = c->fsr_run_idx.alloc((size_t)std::max<int64_t>(n_runs, 1));
+        if (rc == SMOC_OK)
+            rc = c->part.alloc((size_t)std::max<int64_t>(n_runs, 1) * c->Gp * (c->runs ? 8 : 4));
+        if (rc != SMOC_OK)
+            return rc;
+        HIPCHK(hipMemcpy(c->fsr_run_off.p, fro.data(), sizeof(int64_t) * (K + 1),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->fsr_run_idx.p, idx.data(),
+                         sizeof(int64_t) * std::max<int64_t>(n_runs, 1), hipMemcpyHostToDevice));
+        c->run_row.release();
+    } else {
+        // run keys are the (region, interval) rows
+        rc = c->run_row.alloc((size_t)std::max<int64_t>(n_runs, 1));
+        if (rc != SMOC_OK)
+            return rc;
+        if (n_runs > 0)
+            HIPCHK(hipMemcpy(c->run_row.p, run_key.data(), sizeof(int32_t) * n_runs,
+                             hipMemcpyHostToDevice));
+    }
     c->n_runs = n_runs;
     return SMOC_OK;
 }
@@ -834,7 +866,7 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
             return rc;
         HIPCHK(hipMemcpy(c->vol.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
     }
-    if (c->det()) {
+    if (!c->fixed()) {  // (releases what an earlier stream needed when nothing is)
         rc = build_runs(c, S, region, z, T, track_off);
         if (rc != SMOC_OK)
             return rc;
@@ -880,19 +912,28 @@ static void fill_launch(const smoc_ctx *c, const smoc_launch *in, smoc_launch *l
     *l = d;
 }
 
+// Linear chunks (MODE_LIN): a chunked sweep that sums runs in registers (a
+// region-sorted stream, no segment_flush) with a deterministic tally, or an
+// atomic FP32/FP64 one at a chunk length that is a multiple of SMOC_RUN_SPLIT
+// (at other lengths chunks need not begin runs, and the transfer pass runs).
+static bool linear_chunks(const smoc_ctx *c, int chunk, bool segment_flush)
+{
+    return chunk > 0 && c->runs && !segment_flush && !c->fixed() &&
+           (c->det() || chunk % SMOC_RUN_SPLIT == 0);
+}
+
 // Work items of a chunked sweep: every track is cut into chunks of about C
 // segments (smoc_plan.h; an empty track is one empty chunk).  Phase 1 sweeps
 // each track's first chunk (a multi-chunk track's first chunk also emits its
 // psi_out as a transfer) and runs the transfer pass of the middle chunks;
 // phase 2 sweeps the remaining chunks.  Both lists are ordered longest first.
-// An item's first run id (used by the deterministic tally) is its first
-// segment, where every segment is a run, or on a region-sorted stream the run
-// that begins at its first segment: C is then a multiple of SMOC_RUN_SPLIT, so
-// each chunk after a track's first begins at one of the track's split points.
-// There the chunks are linear (MODE_LIN): phase 1 sweeps every chunk, those
-// after a track's first from psi = 0, and list2 names the chunks whose runs
-// k_lin_fix corrects.
-static int build_plan(smoc_ctx *c, int C)
+// An item's first run id is its first segment, where every segment is a run,
+// or with linear chunks the run that begins at its first segment: C is then a
+// multiple of SMOC_RUN_SPLIT, so each chunk after a track's first begins at
+// one of the track's split points.  With linear chunks phase 1 sweeps every
+// chunk, those after a track's first from psi = 0, and list2 names the chunks
+// whose runs k_lin_fix corrects.
+static int build_plan(smoc_ctx *c, int C, bool linear)
 {
     Plan &pl = c->plan;
     // the plan buffers are reallocated below; a graph captured with the old
@@ -901,13 +942,14 @@ static int build_plan(smoc_ctx *c, int C)
     HIPCHK(hipStreamSynchronize(c->stream));
     drop_graph(c);
     pl.chunk = 0;
+    pl.linear = false;
     pl.n_items = pl.n1 = pl.n2 = 0;
     std::vector<int64_t> items, tio((size_t)c->T + 1);
     std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
     const std::vector<int64_t> &off = c->h_track_off;
-    const bool split_ids = c->det() && c->runs;
-    if (split_ids && C % SMOC_RUN_SPLIT != 0) {  // refused by check_launch
-        smoc_set_error("smoc_run: chunk %d is not a multiple of %d", C, SMOC_RUN_SPLIT);
+    const bool split_ids = linear;
+    if (linear && (C % SMOC_RUN_SPLIT != 0 || !c->wrun.p)) {  // refused by check_launch
+        smoc_set_error("smoc_run: no linear chunks of %d segments for this stream", C);
         return SMOC_ERR_ARG;
     }
     int64_t n = 0, sp = 0;
@@ -968,6 +1010,7 @@ static int build_plan(smoc_ctx *c, int C)
     HIPCHK(hipMemcpy(pl.track_item_off.p, tio.data(), sizeof(int64_t) * tio.size(),
                      hipMemcpyHostToDevice));
     pl.chunk = C;
+    pl.linear = linear;
     pl.n_items = n;
     pl.n1 = (int64_t)l1.size();
     pl.n2 = (int64_t)l2.size();
@@ -1005,7 +1048,7 @@ static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *wha
     }
     // linear chunks need wrun, allocated for every stream that can be chunked
     // (a track longer than the chunk length, so longer than SMOC_RUN_SPLIT)
-    if (l.chunk > 0 && c->det() && runs && !c->wrun.p) {
+    if (linear_chunks(c, l.chunk, l.segment_flush != 0) && !c->wrun.p) {
         smoc_set_error("%s: no per-run buffer for chunks of this stream", what);
         return SMOC_ERR_STATE;
     }
@@ -1022,8 +1065,11 @@ static int check_launch(const smoc_ctx *c, const smoc_launch &l, const char *wha
 // work counters (allocated on first use).
 static int ctx_bufs(smoc_ctx *c, const smoc_launch &l, RunBufs *b)
 {
-    if (l.chunk > 0 && c->plan.chunk != l.chunk) {
-        int rc = build_plan(c, l.chunk);
+    // a plan holds linear or transfer entries: rebuilt when the launch needs
+    // the other kind (a segment_flush launch at the same chunk length)
+    const bool linear = linear_chunks(c, l.chunk, l.segment_flush != 0);
+    if (l.chunk > 0 && (c->plan.chunk != l.chunk || c->plan.linear != linear)) {
+        int rc = build_plan(c, l.chunk, linear);
         if (rc != SMOC_OK)
             return rc;
     }
@@ -1108,9 +1154,8 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
                                     c->flux_f64() ? 1 : 0, c->stream));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], c->stream));
-    // linear chunks: a deterministic tally of a region-sorted stream, whose
-    // chunks begin runs (MODE_LIN)
-    const bool linear = chunked && c->det() && runs;
+    // linear chunks: every chunk begins a run (MODE_LIN)
+    const bool linear = linear_chunks(c, l.chunk, l.segment_flush != 0);
     if (chunked) {
         if (b.plan_chunk > 0)
             LAUNCHCHK(smoc_launch_plan(
@@ -1143,8 +1188,9 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
         }
         if (linear) {
             // each run of a chunk after a track's first gets its incoming psi's share
-            smoc_fix_args fa{b.items, b.list2, b.chunk_psi, c->wrun.p, c->part.p,
-                             b.n2, b.n_items, c->n_runs, G, c->Gp, slices};
+            smoc_fix_args fa{b.items, b.list2, b.chunk_psi, c->wrun.p,
+                             c->det() ? (void *)c->part.p : (void *)(c->flux.p + 256),
+                             c->run_row.p, b.n2, b.n_items, c->n_runs, G, c->Gp, slices};
             LAUNCHCHK(smoc_launch_lin_fix(&fa, c->p.tally, gpl, c->stream));
             a.n_items = 0;  // no phase 2
         } else {
@@ -1478,11 +1524,12 @@ int smoc_run_noalloc(smoc_ctx *c, const smoc_launch *launch, smoc_timing *timing
     b.queue = reinterpret_cast<int64_t *>(take(2 * sizeof(int64_t)));
     if (l.chunk > 0) {
         const int64_t C = l.chunk;
+        const bool lin = linear_chunks(c, l.chunk, l.segment_flush != 0);
         int64_t n = 0, n1 = 0, n2 = 0;
         for (int64_t t = 0; t < c->T; t++) {
             const int64_t nc = smoc_chunk_count(c->h_track_off[t], c->h_track_off[t + 1], C);
             n += nc;
-            n1 += c->det() && c->runs ? nc : nc > 1 ? nc - 1 : 1;  // linear chunks: all
+            n1 += lin ? nc : nc > 1 ? nc - 1 : 1;  // linear chunks: all
             n2 += nc - 1;
         }
         const size_t gp = (size_t)c->Gp;

@@ -86,15 +86,18 @@ typedef struct {
 } smoc_scan_args;
 
 /* Correction of linear chunks (k_lin_fix): for every chunk c in list (chunks
- * swept from psi = 0) and every run r of c, part[r] += w chunk_psi[c] wrun[r]
- * (w the track weight), so that the run's partial is its tally from the
- * chunk's true incoming psi. */
+ * swept from psi = 0) and every run r of c, w chunk_psi[c] wrun[r] (w the
+ * track weight) joins the run's tally: added to its partial (deterministic
+ * tallies), or atomically to its (region, interval) row of fine_flux. */
 typedef struct {
     const int64_t *items;          /* [n_items][4]: track, begin, end, first run id */
     const int64_t *list;           /* chunk ids << 2 */
     const float *chunk_psi;        /* [items][Gp] */
     const float *wrun;             /* [runs][Gp] */
-    void *part;                    /* DET_F64: double [runs][Gp]; DET_KAHAN: float2 */
+    void *out;                     /* DET_F64: partials, double [runs][Gp]; DET_KAHAN:
+                                      float2 [runs][Gp]; ATOMIC_F32 / ATOMIC_F64:
+                                      fine_flux [R*F][Gp] */
+    const int32_t *run_row;        /* atomic tallies: the row of each run [runs] */
     int64_t n_list, n_items, n_runs;
     int G, Gp, slices;
 } smoc_fix_args;

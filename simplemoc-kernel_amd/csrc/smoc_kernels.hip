@@ -419,15 +419,15 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // of the middle chunks; phase 2, after the scan, sweeps the remaining chunks.
 // MODE_MIXED kernels pick the mode per work entry from its ITEM_* flags.
 //
-// MODE_LIN (linear chunks; deterministic tallies of a region-sorted stream,
-// whose chunks begin runs): every chunk after a track's first is swept once,
+// MODE_LIN (linear chunks; a region-sorted stream whose chunks begin runs,
+// with any tally but FIXED64): every chunk after a track's first is swept once,
 // in phase 1, from psi = 0.  A segment's tally is affine in the chunk's
 // incoming psi: its psi term is psi_k ev_k / sigT with psi_k = psi0_k +
 // psi_in P_k, P_k the product of 1 - ev over the chunk's earlier segments.
-// The run's partial gets the tally from psi0 and wrun[run] the run's
-// sum of P_k ev_k / sigT; the chunk emits its transfer (psi0_out, P_end).
-// After the scan, k_lin_fix adds psi_in times wrun to each run's partial and
-// k_scan gives the tracks' psi_out.  No segment is swept twice.
+// The run's flush takes the tally from psi0 and wrun[run] the run's sum of
+// P_k ev_k / sigT; the chunk emits its transfer (psi0_out, P_end).  After the
+// scan, k_lin_fix adds psi_in times wrun to each run's tally and k_scan gives
+// the tracks' psi_out.  No segment is swept twice.
 
 enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2, MODE_LIN = 3 };
 enum { ITEM_XFER = 1, ITEM_EMIT = 2, ITEM_LIN = 3 };
@@ -923,7 +923,7 @@ __device__ __forceinline__ void dispatch_item(const smoc_sweep_args &a,
     if constexpr (MODE == MODE_MIXED) {
         const int64_t w = a.slices == 1 ? item : item / a.slices;
         const int flags = (int)(uniform64(a.item_list[w]) & 3);
-        constexpr bool LIN = RUNS && (TALLY == SMOC_TALLY_DET_F64 || TALLY == SMOC_TALLY_DET_KAHAN);
+        constexpr bool LIN = RUNS && TALLY != SMOC_TALLY_FIXED64;
         if (LIN && flags == ITEM_LIN) {
             if constexpr (LIN)
                 sweep_item<STORE, GPL, EXPM, TALLY, RUNS, MODE_LIN>(a, len, code, zin, track_off,
@@ -1061,9 +1061,10 @@ __global__ void __launch_bounds__(256) k_scan(smoc_scan_args a)
 }
 
 // Linear chunks: each run r of a chunk c swept from psi = 0 gets w psi_in(c)
-// wrun[r] added to its partial (w the track weight), in the partial's own
-// arithmetic (FP64, or a compensated FP32 step).  One wave per (list entry,
-// slice); every run belongs to one chunk, so each partial is written once.
+// wrun[r] added (w the track weight): to its partial in the partial's own
+// arithmetic (FP64, or a compensated FP32 step), or with an atomic add to its
+// row of fine_flux.  One wave per (list entry, slice); every run belongs to
+// one chunk, so each partial is written once.
 template <int TALLY, int GPL>
 __global__ void __launch_bounds__(256) k_lin_fix(smoc_fix_args a)
 {
@@ -1090,11 +1091,19 @@ __global__ void __launch_bounds__(256) k_lin_fix(smoc_fix_args a)
         const float pin = a.chunk_psi[c * Gp + g];
         for (int64_t r = r0; r < r1; ++r) {
             const float wr = a.wrun[r * Gp + g];
-            if constexpr (TALLY == SMOC_TALLY_DET_F64) {
-                double *p = static_cast<double *>(a.part) + r * Gp + g;
+            if constexpr (TALLY == SMOC_TALLY_ATOMIC_F32) {
+                const int64_t row = a.run_row[r];
+                unsafeAtomicAdd(static_cast<float *>(a.out) + row * Gp + g,
+                                SMOC_WEIGHT * (pin * wr));
+            } else if constexpr (TALLY == SMOC_TALLY_ATOMIC_F64) {
+                const int64_t row = a.run_row[r];
+                unsafeAtomicAdd(static_cast<double *>(a.out) + row * Gp + g,
+                                (double)SMOC_WEIGHT * ((double)pin * (double)wr));
+            } else if constexpr (TALLY == SMOC_TALLY_DET_F64) {
+                double *p = static_cast<double *>(a.out) + r * Gp + g;
                 *p = *p + (double)SMOC_WEIGHT * ((double)pin * (double)wr);
             } else {
-                float2 *p = static_cast<float2 *>(a.part) + r * Gp + g;
+                float2 *p = static_cast<float2 *>(a.out) + r * Gp + g;
                 float2 v = *p;  // (s, c): s - c, as the run accumulator keeps it
                 const float y = SMOC_WEIGHT * (pin * wr) - v.y;
                 const float s = v.x + y;
@@ -1793,20 +1802,23 @@ extern "C" int smoc_launch_lin_fix(const smoc_fix_args *a, int tally, int gpl, v
     const int64_t n = a->n_list * a->slices;
     if (n == 0)
         return 0;
-    if (tally != SMOC_TALLY_DET_F64 && tally != SMOC_TALLY_DET_KAHAN)
+    if (tally == SMOC_TALLY_FIXED64 || tally < 0 || tally > SMOC_TALLY_FIXED64 ||
+        ((tally == SMOC_TALLY_ATOMIC_F32 || tally == SMOC_TALLY_ATOMIC_F64) && !a->run_row))
         return (int)hipErrorInvalidValue;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const unsigned nb = blocks_for(n, 4);
-    if (tally == SMOC_TALLY_DET_F64) {
+    auto go = [&](auto tc) {
+        constexpr int T = decltype(tc)::value;
         if (gpl == 2)
-            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_F64, 2>), dim3(nb), dim3(256), 0, s, *a);
+            hipLaunchKernelGGL((k_lin_fix<T, 2>), dim3(nb), dim3(256), 0, s, *a);
         else
-            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_F64, 1>), dim3(nb), dim3(256), 0, s, *a);
-    } else {
-        if (gpl == 2)
-            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_KAHAN, 2>), dim3(nb), dim3(256), 0, s, *a);
-        else
-            hipLaunchKernelGGL((k_lin_fix<SMOC_TALLY_DET_KAHAN, 1>), dim3(nb), dim3(256), 0, s, *a);
+            hipLaunchKernelGGL((k_lin_fix<T, 1>), dim3(nb), dim3(256), 0, s, *a);
+    };
+    switch (tally) {
+    case SMOC_TALLY_ATOMIC_F32: go(std::integral_constant<int, SMOC_TALLY_ATOMIC_F32>{}); break;
+    case SMOC_TALLY_ATOMIC_F64: go(std::integral_constant<int, SMOC_TALLY_ATOMIC_F64>{}); break;
+    case SMOC_TALLY_DET_F64: go(std::integral_constant<int, SMOC_TALLY_DET_F64>{}); break;
+    default: go(std::integral_constant<int, SMOC_TALLY_DET_KAHAN>{}); break;
     }
     return (int)hipGetLastError();
 }

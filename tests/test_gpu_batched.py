@@ -111,12 +111,14 @@ def long_runs_oracle(long_runs):
 
 @pytest.mark.parametrize("chunk", [0, 1024, 4096])
 @pytest.mark.parametrize("persistent", [0, 1])
-@pytest.mark.parametrize("tally", ["det-f64", "det-kahan", "fixed64"])
+@pytest.mark.parametrize("tally", ["det-f64", "det-kahan", "fixed64", "atomic-f32",
+                                   "atomic-f64"])
 def test_sorted_chunked_vs_oracle(gpu_ready, long_runs, long_runs_oracle, chunk, persistent,
                                   tally):
-    """Region-sorted stream, tallies that promise bit-exact results, tracks cut
-    at multiples of the chunk length (a multiple of 1024, so every chunk
-    begins a run): against the FP64 oracle."""
+    """Region-sorted stream, tracks cut at split points (the chunk length is a
+    multiple of 1024, so every chunk begins a run): linear chunks for the
+    deterministic and atomic tallies, the transfer pass for FIXED64; against
+    the FP64 oracle."""
     print(f"case sorted chunk={chunk} persistent={persistent} {tally} S={long_runs.S}",
           flush=True)
     flux, psi, tm = sweep(long_runs, tally, chunk=chunk, persistent=persistent)
@@ -150,6 +152,29 @@ def test_sorted_chunked_bit_exact_across_shapes(gpu_ready, long_runs, tally):
         ctx.run()
         whole = ctx.results()
     assert normwise(ref[0], whole[0]) < 2e-6 and normwise(ref[1], whole[1]) < 2e-6
+
+
+@pytest.mark.parametrize("tally", ["atomic-f32", "atomic-f64"])
+def test_sorted_atomic_chunk_schemes(gpu_ready, long_runs, long_runs_oracle, tally):
+    """Atomic tallies on a region-sorted stream: chunks of a multiple of 1024
+    are linear, other lengths take the transfer pass, and a segment_flush
+    launch at the same chunk length gets a plan of its own (its runs are
+    single segments, so no linear chunks) -- each against the FP64 oracle, in
+    an order that switches the plan back and forth."""
+    of, op = long_runs_oracle
+    with smoc.Context(long_runs.G, long_runs.R, long_runs.F, tally=tally) as ctx:
+        ctx.load(long_runs)
+        for launch in ({"chunk": 2048, "persistent": 1}, {"chunk": 1000},
+                       {"chunk": 2048, "segment_flush": 1}, {"chunk": 2048},
+                       {"chunk": 2048, "segment_flush": 1, "use_graph": 1},
+                       {"chunk": 2048, "use_graph": 1}):
+            print(f"case sorted {tally} {launch}", flush=True)
+            tm = ctx.run(**launch)
+            assert tm.used.chunk == launch["chunk"]
+            flux, psi = ctx.results()
+            e = (normwise(flux, of), normwise(psi, op))
+            print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+            assert max(e) < 1e-5, launch
 
 
 def test_sorted_chunk_lengths_refused(gpu_ready, long_runs):

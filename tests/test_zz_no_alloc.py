@@ -56,7 +56,7 @@ def test_noalloc_matches_run_then_smaller_input(gpu_ready, tally):
             check_pair(ctx, tally, launch)
 
 
-@pytest.mark.parametrize("tally", ["det-f64", "fixed64"])
+@pytest.mark.parametrize("tally", ["det-f64", "fixed64", "atomic-f32"])
 def test_noalloc_sorted_chunked(gpu_ready, tally):
     """Region-sorted stream in chunks of a multiple of 1024: the plan built on
     the device takes each chunk's first run id from the run tables as the host
