@@ -46,7 +46,8 @@ says so, timed over the same number of steps):
     the plain kernel on them in generation order (atomic FP32) and the mixed
     path on the same batches sorted (deterministic FP64, long tracks in
     chunks), each autotuned and replayed from a hipGraph; `vs_plain` is the
-    ratio of attenuations/s to the metric's plain line;
+    ratio of attenuations/s to the metric's plain line, and the batched mixed
+    line's error against the FP64 oracle is in `accuracy`;
   * `handover`: the time smoc_set_regions + smoc_set_segments take to bring
     the host buffers in, and the rate of one step including it;
   * `cpu_baseline`: the reference's own attenuate_segment (oracle/_ref) timed
@@ -169,10 +170,11 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
+def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy, batched=None):
     """Reference CPU baseline on a bounded sample (all host threads, then one
     thread on a smaller sample), and the checker: FP64 oracle on the full
-    stream for the accuracy of the GPU results."""
+    stream for the accuracy of the GPU results (batched: the sorted batches
+    and the batched mixed line's results, checked the same way)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
@@ -258,6 +260,10 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy):
                     qf, qp = pyoracle.sweep_f64(quantized(mprob, storage), pyoracle.EXP_NATIVE,
                                                 nthreads=threads)
                     acc[f"{name}_vs_fp64_on_dequantised_inputs"] = errs(f, p, qf, qp)
+        if batched is not None:
+            bprob, (f, p) = batched
+            of, op = pyoracle.sweep_f64(bprob, pyoracle.EXP_NATIVE, nthreads=threads)
+            acc["batched_mixed_vs_fp64"] = errs(f, p, of, op)
         acc["oracle_s"] = time.time() - t0
     return base, acc
 
@@ -350,7 +356,7 @@ def main():
         "input_gen_s": gen_s,
     }
 
-    mixed = None
+    mixed = batched = None
     if not args.no_mixed:
         tuned = {"autotune": 1, "use_graph": 1}
         mwarm = max(args.warmup, 1)
@@ -460,13 +466,13 @@ def main():
                     "segments": problem.S, "tracks": problem.T, "tally": tally,
                     "launch": lused,
                     "checksums": dict(zip(("fine_flux", "state_flux"),
-                                          smoc.checksums(lf, lp)))}
+                                          smoc.checksums(lf, lp)))}, lf, lp
 
-        out["batched_plain"] = bline(
+        out["batched_plain"], _, _ = bline(
             bprob, "atomic-f32",
             f"8192 lognormal track batches ({bprob.S} segments) x {G} groups, FP32, tracks "
             "in generation order, atomic FP32 tally; autotuned launch replayed from a hipGraph")
-        out["batched_mixed"] = bline(
+        out["batched_mixed"], bf, bp = bline(
             bsorted, "det-f64",
             "the same batches, stream pre-sorted by (region, fine interval); FP32 storage and "
             "math, runs summed in registers, deterministic FP64 segmented reduction; long "
@@ -476,12 +482,13 @@ def main():
                                                     out["batched_plain"]["value"])
         out["batched_input"] = {"generate_and_sort_s": bgen_s, "sort_host_s": bsort_s,
                                 "note": "outside every timed region"}
+        batched = (bsorted, (bf, bp))
 
     if not args.no_cpu_baseline:
         log("CPU baseline and FP64 oracle")
         base, acc = cpu_leg(prob, args.cpu_sample_segments, args.cpu_single_segments,
                             (flux, psi), mixed,
-                            not args.no_accuracy)
+                            not args.no_accuracy, batched)
         out["cpu_baseline"] = base
         if acc is not None:
             out["accuracy"] = acc
