@@ -248,3 +248,21 @@ def test_persistent_batched_pulls(gpu_ready, sorted_stream):
     e = (normwise(out[1][0], of), normwise(out[1][1], op))
     print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
     assert max(e) < 1e-5
+
+
+@pytest.mark.parametrize("G", [37, 129, 200])
+@pytest.mark.parametrize("tally", ["det-f64", "atomic-f32"])
+def test_sorted_linear_chunks_group_counts(gpu_ready, G, tally):
+    """Linear chunks with group counts that leave lanes of padding or need
+    several group slices per track (the per-run W stores and the correction
+    skip groups past G), at 1 and 2 groups per lane, against the FP64 oracle."""
+    prob = smoc.generate(0, G=G, R=30, F=5, seed=G, lognormal_tracks=16, median=1500.0,
+                         sigma=1.0, lo=16, hi=5000, sorted_stream=True)
+    of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=8)
+    for gpl in (1, 2):
+        print(f"case linear chunks G={G} {tally} gpl {gpl}", flush=True)
+        flux, psi, tm = sweep(prob, tally, chunk=1024, persistent=1, groups_per_lane=gpl)
+        assert tm.used.chunk == 1024
+        e = (normwise(flux, of), normwise(psi, op))
+        print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+        assert max(e) < 1e-5
