@@ -271,9 +271,9 @@ def cpu_leg(prob, sample_segments, single_segments, gpu_plain, mixed, accuracy, 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                          "(the sweep shards over tracks; the multi-GPU design is in "
-                          "DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (a "
+                          "multi-GPU run would shard the tracks, one context per GPU, and "
+                          "all-reduce fine_flux; that path is not built)"}))
         sys.exit(2)
     if args.copy_probe_mib:
         nbytes = args.copy_probe_mib << 20
