@@ -128,6 +128,7 @@ struct smoc_ctx {
     DevBuf<float> psi_in, psi_out;  // [T][Gp]
     bool segments_set = false;
     bool runs = false;  // sorted-enough stream: sum runs in registers
+    bool split = false;  // runs also end at split points (runs, a track > SMOC_RUN_SPLIT)
     float fix_limit = 0.f;
 
     // deterministic tally bookkeeping
@@ -640,10 +641,7 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     const bool det = c->det();
     // linear chunks need wrun; a stream can be chunked only when a track is
     // longer than the chunk length, which is then at least SMOC_RUN_SPLIT
-    int64_t longest = 0;
-    for (int64_t t = 0; t < T; t++)
-        longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
-    const bool chunkable = c->runs && longest > SMOC_RUN_SPLIT;
+    const bool chunkable = c->split;
     if (!det && !chunkable) {
         c->track_run_off.release();
         c->split_run.release();
@@ -658,7 +656,7 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     std::vector<int64_t> tro((size_t)T + 1), split;
     try {
         run_key.reserve(c->runs ? (size_t)T + 1024 : (size_t)S);
-        if (c->runs)
+        if (chunkable)
             split.reserve((size_t)(S / SMOC_RUN_SPLIT) + 1);
         for (int64_t t = 0; t < T; t++) {
             tro[t] = (int64_t)run_key.size();
@@ -669,7 +667,7 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
                 smoc_decode_z(z[k], F, &f, &zin);
                 const int32_t key = region[k] * F + f;
                 const bool at_split =
-                    c->runs && k > track_off[t] && (k & (SMOC_RUN_SPLIT - 1)) == 0;
+                    chunkable && k > track_off[t] && (k & (SMOC_RUN_SPLIT - 1)) == 0;
                 if (at_split)
                     split.push_back((int64_t)run_key.size());
                 if (!c->runs || key != prev || at_split)
@@ -790,6 +788,12 @@ int smoc_set_segments(smoc_ctx *c, int64_t S, const float *len, const int32_t *r
     // A stream sorted by region keeps most neighbours in the same region;
     // then runs are summed in registers.
     c->runs = S > 1 && 2 * same >= S;
+    {
+        int64_t longest = 0;
+        for (int64_t t = 0; t < T; t++)
+            longest = std::max<int64_t>(longest, track_off[t + 1] - track_off[t]);
+        c->split = c->runs && longest > SMOC_RUN_SPLIT;
+    }
     // FIXED64: no fine_flux sum may reach 2^31, so bound each contribution by
     // 2^31 / (most segments landing in one region interval)
     c->fix_limit = 0.f;
@@ -1127,6 +1131,7 @@ static int enqueue_run(smoc_ctx *c, const smoc_launch &l, const RunBufs &b, hipE
     a.F = c->p.fine_axial;
     a.slices = slices;
     a.runs = runs ? 1 : 0;
+    a.split = c->split ? 1 : 0;
     a.fix_limit = c->fix_limit;
     // persistent grids: as many workgroups as can be resident (8 waves per
     // SIMD, 4 SIMDs per CU)

@@ -21,13 +21,16 @@ extern "C" {
 /* exp table capacity in floats (the reference table has N = 353 intervals) */
 #define SMOC_TABLE_CAP 1024
 
-/* On a region-sorted stream a run (the stretch of one track in one region
+/* On a region-sorted stream that can be chunked (a track is longer than
+ * SMOC_RUN_SPLIT segments), a run (the stretch of one track in one region
  * interval whose tally is summed in registers) also ends before every stream
  * index that is a multiple of SMOC_RUN_SPLIT (a split point).  For a chunk
  * length C that is a multiple of SMOC_RUN_SPLIT, chunked sweeps cut tracks at
  * split points only (smoc_plan.h), so every chunk begins a run, and the run
  * partition, with it every deterministic and FIXED64 sum, is the same whether
- * the tracks are chunked or not.  A power of two. */
+ * the tracks are chunked or not.  A stream whose tracks are all shorter cannot
+ * be chunked (a chunk at least as long as every track turns chunking off), and
+ * its runs are not split.  A power of two. */
 #define SMOC_RUN_SPLIT 1024
 
 void smoc_set_error(const char *fmt, ...);
@@ -68,6 +71,8 @@ typedef struct {
     float tab_dx, tab_max;
     int tab_n;
     int G, Gp, F, slices;        /* Gp: padded row stride (multiple of 128) */
+    int split;                   /* 1: runs also end at split points (a stream
+                                    that can be chunked, see SMOC_RUN_SPLIT) */
     int runs;                    /* 1: sum runs of equal (region, interval) in
                                     registers; 0: every segment is a run */
     float fix_limit;             /* FIXED64: bound on one segment's tally */

@@ -707,22 +707,24 @@ __device__ __forceinline__ void sweep_item(const smoc_sweep_args &a,
         // device arrays are padded, so loads past the item's end are in
         // bounds; blocks never reach past it), and up to kSeries consecutive
         // segments that all take the series branch have their psi-free parts
-        // evaluated together before their psi updates run in order.  A run
-        // also ends before every stream index that is a multiple of
-        // SMOC_RUN_SPLIT (a split point; the run table of the deterministic
-        // tally splits there too).  Chunks of a multiple of that length are
-        // cut at split points, so each begins a run, and the blocks and the
-        // sums do not depend on the chunking.
+        // evaluated together before their psi updates run in order.  On a
+        // stream that can be chunked (a.split) a run also ends before every
+        // stream index that is a multiple of SMOC_RUN_SPLIT (a split point;
+        // the run table splits there too).  Chunks of a multiple of that
+        // length are cut at split points, so each begins a run, and the blocks
+        // and the sums do not depend on the chunking.
         if (kb32 < ke32) {
             RunK rk[GPL];
             uint32_t k = kb32;
             RawSeg cur = load_raw(len, code, zin, k);
             for (;;) {
                 const int ccode = cur.code;
-                // the run ends by rend at the latest: the item's end or the next
-                // split point (one min per run: capping each block by both would
-                // compile to a vector min3 of scalars)
-                const uint32_t rend = min(ke32, (k | (SMOC_RUN_SPLIT - 1)) + 1);
+                // the run ends by rend at the latest: the item's end or, on a
+                // stream that can be chunked, the next split point (one min per
+                // run: capping each block by both would compile to a vector
+                // min3 of scalars)
+                const uint32_t rend =
+                    a.split ? min(ke32, (k | (SMOC_RUN_SPLIT - 1)) + 1) : ke32;
                 {
                     const raw_t *__restrict__ base = tup + (int64_t)(ccode >> 2) * Gp;
                     raw_t b[GPL];
