@@ -74,11 +74,11 @@ extern "C" {
  *               segments join the FP64 run sum) and stored; a segmented
  *               reduction then sums the runs of every (region, interval) in
  *               a fixed order.  Bit-exact run to run and across launch
- *               shapes.  On a region-sorted stream a run also ends before
- *               every stream index that is a multiple of 1024, and chunked
- *               sweeps need a chunk length that is a multiple of 1024
- *               (others would split runs: SMOC_ERR_ARG); tracks are then cut
- *               at such indices only.
+ *               shapes.  On a region-sorted stream with a track longer than
+ *               1024 segments a run also ends before every stream index that
+ *               is a multiple of 1024, and chunked sweeps need a chunk length
+ *               that is a multiple of 1024 (others would split runs:
+ *               SMOC_ERR_ARG); tracks are then cut at such indices only.
  *   DET_KAHAN   as DET_F64 with compensated (Kahan) FP32 sums (of the
  *               FP32 block sums, on a region-sorted stream).
  *   FIXED64     deterministic: each run is rounded to a 64-bit fixed-point
