@@ -266,3 +266,28 @@ def test_sorted_linear_chunks_group_counts(gpu_ready, G, tally):
         e = (normwise(flux, of), normwise(psi, op))
         print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
         assert max(e) < 1e-5
+
+
+@pytest.mark.parametrize("tally", ["det-f64", "atomic-f32"])
+def test_sorted_linear_chunks_with_empty_tracks(gpu_ready, long_runs, tally):
+    """Empty tracks between long ones: an empty track's single empty chunk
+    holds the run id where the previous track's runs end, which bounds the
+    last chunk's runs in the per-run correction; results against the FP64
+    oracle, psi_out of the empty tracks equal to their psi_in."""
+    import copy
+    prob = copy.copy(long_runs)
+    off = long_runs.track_off
+    # an empty track before the first, after every fifth, and at the end
+    cuts = [0] + [int(x) for x in off[1:-1:5]] + [int(off[-1])]
+    prob.track_off = np.sort(np.concatenate([off, np.asarray(cuts, np.int64)]))
+    T = prob.track_off.size - 1
+    prob.psi_in = np.random.default_rng(4).random((T, prob.G), dtype=np.float32)
+    of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=8)
+    print(f"case linear chunks with empty tracks {tally}: T {T}", flush=True)
+    flux, psi, tm = sweep(prob, tally, chunk=1024, persistent=1)
+    assert tm.used.chunk == 1024
+    e = (normwise(flux, of), normwise(psi, op))
+    print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+    assert max(e) < 1e-5
+    empty = np.flatnonzero(np.diff(prob.track_off) == 0)
+    assert empty.size >= 3 and np.array_equal(psi[empty], prob.psi_in[empty])
