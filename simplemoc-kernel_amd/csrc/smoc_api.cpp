@@ -704,15 +704,20 @@ static int build_runs(smoc_ctx *c, int64_t S, const int32_t *region, const float
     HIPCHK(hipMemcpy(c->track_run_off.p, tro.data(), sizeof(int64_t) * (T + 1),
                      hipMemcpyHostToDevice));
     if (det) {
-        std::vector<int64_t> fro((size_t)K + 1, 0), idx((size_t)std::max<int64_t>(n_runs, 1));
-        for (int64_t r = 0; r < n_runs; r++)
-            fro[run_key[r] + 1]++;
-        for (int64_t k = 0; k < K; k++)
-            fro[k + 1] += fro[k];
-        {
+        std::vector<int64_t> fro, idx;
+        try {
+            fro.assign((size_t)K + 1, 0);
+            idx.resize((size_t)std::max<int64_t>(n_runs, 1));
+            for (int64_t r = 0; r < n_runs; r++)
+                fro[run_key[r] + 1]++;
+            for (int64_t k = 0; k < K; k++)
+                fro[k + 1] += fro[k];
             std::vector<int64_t> pos(fro.begin(), fro.end() - 1);
             for (int64_t r = 0; r < n_runs; r++)
                 idx[pos[run_key[r]]++] = r;
+        } catch (...) {
+            smoc_set_error("smoc_set_segments: host allocation for run bookkeeping failed");
+            return SMOC_ERR_NOMEM;
         }
         rc = c->fsr_run_off.alloc((size_t)K + 1);
         if (rc == SMOC_OK)
@@ -948,47 +953,52 @@ static int build_plan(smoc_ctx *c, int C, bool linear)
     pl.chunk = 0;
     pl.linear = false;
     pl.n_items = pl.n1 = pl.n2 = 0;
-    std::vector<int64_t> items, tio((size_t)c->T + 1);
-    std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
-    const std::vector<int64_t> &off = c->h_track_off;
-    const bool split_ids = linear;
     if (linear && (C % SMOC_RUN_SPLIT != 0 || !c->wrun.p)) {  // refused by check_launch
         smoc_set_error("smoc_run: no linear chunks of %d segments for this stream", C);
         return SMOC_ERR_ARG;
     }
-    int64_t n = 0, sp = 0;
-    for (int64_t t = 0; t < c->T; t++) {
-        tio[t] = n;
-        const int64_t b = off[t], e = off[t + 1];
-        const int64_t nc = smoc_chunk_count(b, e, C);
-        for (int64_t k = 0; k < nc; k++, n++) {
-            const int64_t cb = smoc_chunk_begin(b, k, C);
-            const int64_t ce = smoc_chunk_end(b, e, k, C);
-            items.push_back(t);
-            items.push_back(cb);
-            items.push_back(ce);
-            items.push_back(!split_ids ? cb
-                            : k == 0   ? c->h_track_run_off[t]
-                                       : c->h_split_run[sp + smoc_split_index(b, cb) - 1]);
-            if (k == 0)
-                p1.emplace_back(ce - cb, (n << 2) | (nc > 1 ? 2 : 0));
-            else
-                p2.emplace_back(ce - cb, n << 2);
-            if (k > 0 && (split_ids || k + 1 < nc))  // linear chunk (3) or transfer (1)
-                p1.emplace_back(ce - cb, (n << 2) | (split_ids ? 3 : 1));
+    std::vector<int64_t> items, tio, l1, l2;
+    int64_t n = 0;
+    try {
+        tio.resize((size_t)c->T + 1);
+        std::vector<std::pair<int64_t, int64_t>> p1, p2;  // (length, entry)
+        const std::vector<int64_t> &off = c->h_track_off;
+        int64_t sp = 0;  // the track's first entry in the split-point table
+        for (int64_t t = 0; t < c->T; t++) {
+            tio[t] = n;
+            const int64_t b = off[t], e = off[t + 1];
+            const int64_t nc = smoc_chunk_count(b, e, C);
+            for (int64_t k = 0; k < nc; k++, n++) {
+                const int64_t cb = smoc_chunk_begin(b, k, C);
+                const int64_t ce = smoc_chunk_end(b, e, k, C);
+                items.push_back(t);
+                items.push_back(cb);
+                items.push_back(ce);
+                items.push_back(!linear ? cb
+                                : k == 0 ? c->h_track_run_off[t]
+                                         : c->h_split_run[sp + smoc_split_index(b, cb) - 1]);
+                if (k == 0)
+                    p1.emplace_back(ce - cb, (n << 2) | (nc > 1 ? SMOC_ITEM_EMIT : 0));
+                else
+                    p2.emplace_back(ce - cb, n << 2);
+                if (k > 0 && (linear || k + 1 < nc))
+                    p1.emplace_back(ce - cb, (n << 2) | (linear ? SMOC_ITEM_LIN : SMOC_ITEM_XFER));
+            }
+            sp += smoc_split_count(b, e);
         }
-        sp += smoc_split_count(b, e);
+        tio[c->T] = n;
+        auto longest_first = [](const std::pair<int64_t, int64_t> &x,
+                                const std::pair<int64_t, int64_t> &y) { return x.first > y.first; };
+        std::stable_sort(p1.begin(), p1.end(), longest_first);
+        std::stable_sort(p2.begin(), p2.end(), longest_first);
+        for (auto &x : p1)
+            l1.push_back(x.second);
+        for (auto &x : p2)
+            l2.push_back(x.second);
+    } catch (...) {
+        smoc_set_error("smoc_run: host allocation for the chunk plan failed");
+        return SMOC_ERR_NOMEM;
     }
-    tio[c->T] = n;
-    auto longest_first = [](const std::pair<int64_t, int64_t> &x,
-                            const std::pair<int64_t, int64_t> &y) { return x.first > y.first; };
-    std::stable_sort(p1.begin(), p1.end(), longest_first);
-    std::stable_sort(p2.begin(), p2.end(), longest_first);
-    std::vector<int64_t> l1, l2;
-    for (auto &x : p1)
-        l1.push_back(x.second);
-    for (auto &x : p2)
-        l2.push_back(x.second);
     int rc = pl.items.alloc(std::max<size_t>(items.size(), 4));
     if (rc == SMOC_OK)
         rc = pl.list1.alloc(std::max<size_t>(l1.size(), 1));

@@ -33,6 +33,11 @@ extern "C" {
  * its runs are not split.  A power of two. */
 #define SMOC_RUN_SPLIT 1024
 
+/* flags of a chunked sweep's work-list entries (entry = item id << 2 | flags) */
+#define SMOC_ITEM_XFER 1 /* transfer pass of a middle chunk                     */
+#define SMOC_ITEM_EMIT 2 /* sweep of a first chunk that emits its psi_out       */
+#define SMOC_ITEM_LIN 3  /* linear chunk: swept from psi = 0, emits its transfer */
+
 void smoc_set_error(const char *fmt, ...);
 
 /* Arguments of the attenuation sweep.  One work item is (track or chunk,

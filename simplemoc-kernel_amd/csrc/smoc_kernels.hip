@@ -430,7 +430,7 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 // the tracks' psi_out.  No segment is swept twice.
 
 enum { MODE_SWEEP = 0, MODE_XFER = 1, MODE_MIXED = 2, MODE_LIN = 3 };
-enum { ITEM_XFER = 1, ITEM_EMIT = 2, ITEM_LIN = 3 };
+enum { ITEM_XFER = SMOC_ITEM_XFER, ITEM_EMIT = SMOC_ITEM_EMIT, ITEM_LIN = SMOC_ITEM_LIN };
 
 // Per-lane constants of one run (a track's consecutive segments in one
 // (region, interval)): everything of kernel.c:111-332 that depends on the
