@@ -291,3 +291,33 @@ def test_sorted_linear_chunks_with_empty_tracks(gpu_ready, long_runs, tally):
     assert max(e) < 1e-5
     empty = np.flatnonzero(np.diff(prob.track_off) == 0)
     assert empty.size >= 3 and np.array_equal(psi[empty], prob.psi_in[empty])
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("tally", ["det-f64", "atomic-f32"])
+def test_sorted_linear_chunks_16bit_storage(gpu_ready, long_runs, storage, tally):
+    """Linear chunks with 16-bit sigT / fine_source, against the FP64 oracle
+    on the dequantised inputs (the arithmetic error)."""
+    from conftest import quantized
+    of, op = pyoracle.sweep_f64(quantized(long_runs, storage), pyoracle.EXP_NATIVE, nthreads=8)
+    print(f"case linear chunks {storage} {tally}", flush=True)
+    flux, psi, tm = sweep(long_runs, tally, storage=storage, chunk=2048, persistent=1)
+    assert tm.used.chunk == 2048
+    e = (normwise(flux, of), normwise(psi, op))
+    print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+    assert max(e) < 1e-5
+
+
+@pytest.mark.parametrize("F", [2, 3, 8])
+def test_sorted_linear_chunks_fine_intervals(gpu_ready, F):
+    """Linear chunks for F = 2 (every row a first or last interval, so every
+    run takes the linear fit), 3 and 8, against the FP64 oracle."""
+    prob = smoc.generate(0, G=100, R=40, F=F, seed=60 + F, lognormal_tracks=24, median=1500.0,
+                         sigma=1.0, lo=16, hi=6000, sorted_stream=True)
+    of, op = pyoracle.sweep_f64(prob, pyoracle.EXP_NATIVE, nthreads=8)
+    print(f"case linear chunks F={F}", flush=True)
+    flux, psi, tm = sweep(prob, "det-f64", chunk=1024, persistent=1)
+    assert tm.used.chunk == 1024
+    e = (normwise(flux, of), normwise(psi, op))
+    print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+    assert max(e) < 1e-5
