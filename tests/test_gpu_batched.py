@@ -321,3 +321,22 @@ def test_sorted_linear_chunks_fine_intervals(gpu_ready, F):
     e = (normwise(flux, of), normwise(psi, op))
     print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
     assert max(e) < 1e-5
+
+
+@pytest.mark.parametrize("tally", ["det-f64", "atomic-f32", "atomic-f64"])
+def test_sorted_linear_chunks_with_init(gpu_ready, long_runs, tally):
+    """Linear chunks accumulating onto a fine_flux_init: the deterministic
+    reduction adds it, the atomic tallies start from it and the per-run
+    correction adds onto it; against the FP64 oracle with the same init."""
+    init = np.random.default_rng(8).random((long_runs.R, long_runs.F, long_runs.G),
+                                           dtype=np.float32)
+    of, op = pyoracle.sweep_f64(long_runs, pyoracle.EXP_NATIVE, nthreads=8, fine_flux_init=init)
+    print(f"case linear chunks with init {tally}", flush=True)
+    with smoc.Context(long_runs.G, long_runs.R, long_runs.F, tally=tally) as ctx:
+        ctx.load(long_runs, init)
+        tm = ctx.run(chunk=1024, persistent=1, use_graph=1)
+        assert tm.used.chunk == 1024
+        flux, psi = ctx.results()
+    e = (normwise(flux, of), normwise(psi, op))
+    print(f"  normwise {e[0]:.2e} / {e[1]:.2e}", flush=True)
+    assert max(e) < 1e-5
